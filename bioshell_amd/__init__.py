@@ -1,0 +1,23 @@
+"""MI355X all-vs-all optimal-superposition cRMSD for BioShell ensembles.
+
+Hand-written HIP kernels for gfx950 behind the C ABI of
+``include/bioshell_gpu.h``; this package is the Python host mirror of that ABI.
+"""
+from .crmsd import (  # noqa: F401
+    BioshellGpuError,
+    Context,
+    Method,
+    RmsdMatrix,
+    Rototranslation,
+    check,
+    crmsd,
+    crmsd_all_vs_all,
+    crmsd_all_vs_all_device,
+    crmsd_all_vs_all_streamed,
+    default_context,
+    device_count,
+    generate_ensemble,
+    superpose,
+    tri_index,
+    version,
+)

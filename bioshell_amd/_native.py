@@ -1,0 +1,96 @@
+"""Loads ``libbioshell_gpu.so`` and declares the C ABI of include/bioshell_gpu.h.
+
+The library must have been built (``__graft_entry__.build()`` or
+``python -m bioshell_amd.build``); there is no fallback implementation, so a
+missing library is an error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbioshell_gpu.so")
+
+c_status = ctypes.c_int32
+c_dp = ctypes.POINTER(ctypes.c_double)
+c_fp = ctypes.POINTER(ctypes.c_float)
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+c_vp = ctypes.c_void_p
+c_size = ctypes.c_size_t
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("total_ms", ctypes.c_double),
+        ("prep_ms", ctypes.c_double),
+        ("main_ms", ctypes.c_double),
+        ("fallback_ms", ctypes.c_double),
+        ("pairs", ctypes.c_uint64),
+        ("flagged_pairs", ctypes.c_uint64),
+        ("main_launches", ctypes.c_uint64),
+        ("tile_m", ctypes.c_int32),
+        ("tile_n", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# name -> (restype, argtypes); every symbol declared in include/bioshell_gpu.h
+SIGNATURES = {
+    "bs_last_error": (ctypes.c_char_p, []),
+    "bs_version": (ctypes.c_char_p, []),
+    "bs_device_count": (ctypes.c_int, []),
+    "bs_context_create": (c_status, [ctypes.c_int, c_size, ctypes.POINTER(c_vp)]),
+    "bs_context_destroy": (None, [c_vp]),
+    "bs_context_synchronize": (c_status, [c_vp]),
+    "bs_context_stream": (c_vp, [c_vp]),
+    "bs_context_last_stats": (c_status, [c_vp, ctypes.POINTER(RunStats)]),
+    "bs_context_bytes_in_use": (c_size, [c_vp]),
+    "bs_crmsd_pair_f64": (c_status, [c_vp, c_dp, c_dp, c_size, c_dp]),
+    "bs_superpose_f64": (c_status, [c_vp, c_dp, c_dp, c_size, c_dp, c_dp, c_dp]),
+    "bs_crmsd_all_vs_all": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_vp, ctypes.c_int]),
+    "bs_crmsd_all_vs_all_device": (c_status, [c_vp, c_vp, c_size, c_size, ctypes.c_int, c_vp, ctypes.c_int]),
+    "bs_crmsd_all_vs_all_streamed": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_size, c_fp]),
+    "bs_crmsd_batch": (c_status, [c_vp, ctypes.POINTER(c_dp), ctypes.POINTER(c_size), ctypes.POINTER(c_size),
+                                  c_size, ctypes.c_int, ctypes.POINTER(c_fp)]),
+    "bs_autotune": (c_status, [c_vp, ctypes.c_int, ctypes.c_char_p]),
+    "bs_autotune_load": (c_status, [c_vp, ctypes.c_char_p]),
+    "bs_library_create": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.POINTER(c_vp)]),
+    "bs_library_destroy": (None, [c_vp]),
+    "bs_library_size": (c_size, [c_vp]),
+    "bs_library_atoms": (c_size, [c_vp]),
+    "bs_query_topk": (c_status, [c_vp, c_vp, c_dp, c_size, c_size, c_u32p, c_fp]),
+    "bs_query_all": (c_status, [c_vp, c_vp, c_dp, c_size, c_fp]),
+    "bs_server_create": (c_status, [c_vp, ctypes.POINTER(c_vp), c_size, c_size, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "bs_server_destroy": (None, [c_vp]),
+    "bs_server_submit": (c_status, [c_vp, c_size, c_dp, c_size, c_size, c_u32p, c_fp, c_u64p]),
+    "bs_server_poll": (c_status, [c_vp, ctypes.c_uint64]),
+    "bs_server_wait": (c_status, [c_vp, ctypes.c_uint64]),
+    "bs_server_cancel": (c_status, [c_vp, ctypes.c_uint64]),
+    "bs_server_latency_ms": (c_status, [c_vp, ctypes.c_uint64, c_dp]),
+    "bs_generate_ensemble": (c_status, [ctypes.c_uint64, c_size, c_size, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, c_dp]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()')"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,219 @@
+// api.cpp -- the extern "C" entry points declared in include/bioshell_gpu.h.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "engine.hpp"
+
+namespace bs {
+const char* last_error_cstr();
+}
+
+using namespace bs;
+
+namespace {
+
+void check_ctx(const bs_context* ctx) { BS_REQUIRE(ctx != nullptr, "context is null"); }
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        BS_HIP(hipGetDevice(&prev));
+        if (prev != dev) BS_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+void check_shape(size_t n_conf, size_t n_atoms)
+{
+    BS_REQUIRE(n_atoms >= 1, "n_atoms must be >= 1 (got %zu)", n_atoms);
+    BS_REQUIRE(n_atoms <= (1u << 20), "n_atoms too large (%zu)", n_atoms);
+    BS_REQUIRE(n_conf <= (1u << 21), "n_conf too large (%zu)", n_conf);
+}
+
+bool valid_method(bs_method m) { return m >= BS_FP64_PLAIN && m <= BS_BF16X3_GRAM; }
+
+}  // namespace
+
+extern "C" {
+
+const char* bs_last_error(void) { return bs::last_error_cstr(); }
+
+const char* bs_version(void) { return "bioshell-mi355x 0.1.0 (gfx950)"; }
+
+int bs_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** out)
+{
+    return guard([&] {
+        BS_REQUIRE(out != nullptr, "out is null");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) fail(BS_ERR_NO_DEVICE, "no HIP device visible");
+        BS_REQUIRE(device >= 0 && device < n, "device %d out of range (%d visible)", device, n);
+        DeviceGuard dg(device);
+        hipDeviceProp_t prop;
+        BS_HIP(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            fail(BS_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950", device,
+                 prop.gcnArchName);
+        auto ctx = std::make_unique<bs_context>();
+        ctx->device = device;
+        ctx->n_cu = prop.multiProcessorCount;
+        ctx->budget.set_cap(hbm_budget_bytes);
+        BS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->prep.bind(&ctx->budget);
+        ctx->gsc.bind(&ctx->budget);
+        ctx->ws_xyz.bind(&ctx->budget);
+        ctx->ws_out.bind(&ctx->budget);
+        ctx->ws_misc.bind(&ctx->budget);
+        ctx->t_total.create();
+        ctx->t_prep.create();
+        ctx->t_main.create();
+        ctx->t_fallback.create();
+        *out = ctx.release();
+    });
+}
+
+void bs_context_destroy(bs_context* ctx)
+{
+    if (!ctx) return;
+    (void)guard([&] {
+        DeviceGuard dg(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->prep.reset();
+        ctx->t_total.destroy();
+        ctx->t_prep.destroy();
+        ctx->t_main.destroy();
+        ctx->t_fallback.destroy();
+        if (ctx->gsc.h_count) (void)hipHostFree(ctx->gsc.h_count);
+        ctx->gsc.h_count = nullptr;
+        ctx->gsc.flags.reset();
+        ctx->gsc.tiles.reset();
+        ctx->ws_xyz.reset();
+        ctx->ws_out.reset();
+        ctx->ws_misc.reset();
+        (void)hipStreamDestroy(ctx->stream);
+    });
+    delete ctx;
+}
+
+bs_status bs_context_synchronize(bs_context* ctx)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        BS_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void* bs_context_stream(bs_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+bs_status bs_context_last_stats(const bs_context* ctx, bs_run_stats* out)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        BS_REQUIRE(out != nullptr, "out is null");
+        *out = ctx->last;
+    });
+}
+
+size_t bs_context_bytes_in_use(const bs_context* ctx) { return ctx ? ctx->budget.in_use() : 0; }
+
+bs_status bs_crmsd_pair_f64(bs_context* ctx, const double* a_xyz, const double* b_xyz,
+                            size_t n_atoms, double* out)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        BS_REQUIRE(a_xyz && b_xyz && out, "null pointer argument");
+        check_shape(2, n_atoms);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        hipStream_t s = ctx->stream;
+        double* d = static_cast<double*>(ctx->ws_xyz.ensure(n_atoms * 6 * 8, "pair coordinates"));
+        double* o = static_cast<double*>(ctx->ws_out.ensure(64, "pair output"));
+        BS_HIP(hipMemcpyAsync(d, a_xyz, n_atoms * 24, hipMemcpyHostToDevice, s));
+        BS_HIP(hipMemcpyAsync(d + 3 * n_atoms, b_xyz, n_atoms * 24, hipMemcpyHostToDevice, s));
+        avsa_device(ctx, d, 2, (int)n_atoms, BS_FP64_PLAIN, o, true);
+        BS_HIP(hipMemcpyAsync(out, o, 8, hipMemcpyDeviceToHost, s));
+        BS_HIP(hipStreamSynchronize(s));
+    });
+}
+
+bs_status bs_superpose_f64(bs_context* ctx, const double* a_xyz, const double* b_xyz,
+                           size_t n_atoms, double* rot9, double* trans3, double* rmsd)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        BS_REQUIRE(a_xyz && b_xyz && rot9 && trans3, "null pointer argument");
+        check_shape(2, n_atoms);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        hipStream_t s = ctx->stream;
+        double* d = static_cast<double*>(ctx->ws_xyz.ensure(n_atoms * 6 * 8, "pair coordinates"));
+        double* o = static_cast<double*>(ctx->ws_out.ensure(16 * 8, "superposition output"));
+        BS_HIP(hipMemcpyAsync(d, a_xyz, n_atoms * 24, hipMemcpyHostToDevice, s));
+        BS_HIP(hipMemcpyAsync(d + 3 * n_atoms, b_xyz, n_atoms * 24, hipMemcpyHostToDevice, s));
+        launch_superpose(d, d + 3 * n_atoms, (int)n_atoms, o, s);
+        BS_HIP(hipGetLastError());
+        double h[13];
+        BS_HIP(hipMemcpyAsync(h, o, sizeof(h), hipMemcpyDeviceToHost, s));
+        BS_HIP(hipStreamSynchronize(s));
+        std::memcpy(rot9, h, 9 * 8);
+        std::memcpy(trans3, h + 9, 3 * 8);
+        if (rmsd) *rmsd = h[12];
+    });
+}
+
+bs_status bs_crmsd_all_vs_all_device(bs_context* ctx, const double* d_xyz, size_t n_conf,
+                                     size_t n_atoms, bs_method method, void* d_out_upper,
+                                     int out_f64)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        check_shape(n_conf, n_atoms);
+        BS_REQUIRE(valid_method(method), "unknown method %d", (int)method);
+        BS_REQUIRE(n_conf < 2 || (d_xyz && d_out_upper), "null pointer argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        avsa_device(ctx, d_xyz, (int)n_conf, (int)n_atoms, method, d_out_upper, out_f64 != 0);
+    });
+}
+
+bs_status bs_crmsd_all_vs_all(bs_context* ctx, const double* xyz, size_t n_conf, size_t n_atoms,
+                              bs_method method, void* out_upper, int out_f64)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        check_shape(n_conf, n_atoms);
+        BS_REQUIRE(valid_method(method), "unknown method %d", (int)method);
+        if (n_conf < 2) {
+            ctx->last = bs_run_stats{};
+            return;
+        }
+        BS_REQUIRE(xyz && out_upper, "null pointer argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        hipStream_t s = ctx->stream;
+        const size_t in_bytes = n_conf * n_atoms * 3 * 8;
+        const size_t pairs = n_conf * (n_conf - 1) / 2;
+        const size_t out_bytes = pairs * (out_f64 ? 8 : 4);
+        double* d = static_cast<double*>(ctx->ws_xyz.ensure(in_bytes, "ensemble coordinates"));
+        void* o = ctx->ws_out.ensure(out_bytes, "rmsd matrix");
+        BS_HIP(hipMemcpyAsync(d, xyz, in_bytes, hipMemcpyHostToDevice, s));
+        avsa_device(ctx, d, (int)n_conf, (int)n_atoms, method, o, out_f64 != 0);
+        BS_HIP(hipMemcpyAsync(out_upper, o, out_bytes, hipMemcpyDeviceToHost, s));
+        BS_HIP(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// engine.cpp -- the all-vs-all pipeline: preparation, Gram launch, FP64
+// re-evaluation of flagged pairs, statistics.
+#include "engine.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace bs {
+
+GramMode mode_of(bs_method m) { return m == BS_FP32_GRAM ? kGramFP32 : kGramBF16x3; }
+bool is_gram(bs_method m) { return m == BS_FP32_GRAM || m == BS_BF16X3_GRAM; }
+
+static constexpr int kTileRows = 64;  // pairs per work-group tile side
+
+size_t prepared_bytes(int n, int L)
+{
+    const size_t n_pad = (size_t)round_up(n, kTileRows);
+    const size_t n_kblk = (size_t)(L + kAtomBlock - 1) / kAtomBlock;
+    const size_t slices = (size_t)(n + 63) / 64;
+    return (size_t)n * L * 3 * 8 + (size_t)n * 9 * 8 + (size_t)(3 * L + 6) * 8 +
+           slices * 3 * L * 8 + n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16 +
+           (size_t)kNumConsts * n_pad * 8;
+}
+
+void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
+             const double* fixed_ref, hipStream_t s)
+{
+    P.n = n;
+    P.L = L;
+    P.n_pad = round_up(n, kTileRows);
+    P.n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
+    P.mode = mode;
+    const int slices = (n + 63) / 64;
+    double* centred = static_cast<double*>(P.centred.ensure((size_t)n * L * 3 * 8, "centred coordinates"));
+    double* rot = static_cast<double*>(P.rot.ensure((size_t)n * 9 * 8, "rotations"));
+    double* ref = static_cast<double*>(P.ref.ensure((size_t)(3 * L + 6) * 8, "reference"));
+    double* partial = static_cast<double*>(P.partial.ensure((size_t)slices * 3 * L * 8, "reference partial sums"));
+    void* panels = P.panels.ensure((size_t)P.n_pad / kConfBlock * P.n_kblk * kPanelBlockU4 * 16,
+                                   "operand panels");
+    double* consts = static_cast<double*>(P.consts.ensure((size_t)kNumConsts * P.n_pad * 8, "constants"));
+
+    launch_centre(d_xyz, n, L, centred, s);
+    if (fixed_ref) {
+        BS_HIP(hipMemcpyAsync(ref, fixed_ref, (size_t)(3 * L + 6) * 8, hipMemcpyDeviceToDevice, s));
+        launch_fit(centred, n, L, ref, rot, s);
+    } else {
+        launch_ref_from_first(centred, L, ref, s);
+        launch_fit(centred, n, L, ref, rot, s);
+        launch_mean(centred, rot, n, L, partial, ref, s);
+        launch_fit(centred, n, L, ref, rot, s);
+    }
+    launch_pack(centred, rot, ref, n, P.n_pad, L, P.n_kblk, mode, panels, consts, P.n_pad, s);
+    BS_HIP(hipGetLastError());
+}
+
+uint64_t run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
+                  hipStream_t s, Timer* t_main, Timer* t_fb, int* launches)
+{
+    BS_REQUIRE(A.L == B.L && A.mode == B.mode, "prepared sets disagree on atom count or mode");
+    const int tiles_r = A.n_pad / kTileRows, tiles_c = B.n_pad / kTileRows;
+    BS_REQUIRE(tiles_r < 65536 && tiles_c < 65536, "too many conformations for one launch");
+    const uint64_t key = ((uint64_t)tiles_r << 33) | ((uint64_t)tiles_c << 1) | (run.triangle ? 1 : 0);
+    if (key != sc.tiles_key) {
+        const size_t cap = (size_t)tiles_r * tiles_c;
+        std::vector<uint32_t> h(cap);
+        const int nt = build_tile_order(tiles_r, tiles_c, run.triangle, 8, h.data());
+        uint32_t* d = static_cast<uint32_t*>(sc.tiles.ensure((size_t)nt * 4 + 4, "tile order"));
+        BS_HIP(hipMemcpyAsync(d, h.data(), (size_t)nt * 4, hipMemcpyHostToDevice, s));
+        BS_HIP(hipStreamSynchronize(s));  // h goes out of scope
+        sc.tiles_key = key;
+        sc.tiles_n = nt;
+    }
+    const uint64_t pairs = run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * B.n;
+    sc.max_flagged = (int)std::max<uint64_t>(
+        (uint64_t)sc.max_flagged, std::max<uint64_t>(4096, std::min<uint64_t>(pairs, 1u << 22)));
+    if (!sc.h_count) BS_HIP(hipHostMalloc((void**)&sc.h_count, 64, hipHostMallocDefault));
+
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        uint32_t* flags = static_cast<uint32_t*>(
+            sc.flags.ensure((size_t)(2 * (size_t)sc.max_flagged + 64) * 4, "flagged pairs"));
+        uint32_t* count = flags + 2 * (size_t)sc.max_flagged;
+        launch_zero_u32(count, 1, s);
+
+        GramArgs g{};
+        g.panels_a = A.panels.get();
+        g.panels_b = B.panels.get();
+        g.consts_a = A.consts.as<double>();
+        g.consts_b = B.consts.as<double>();
+        g.lda = A.n_pad;
+        g.ldb = B.n_pad;
+        g.n_rows = A.n;
+        g.n_cols = B.n;
+        g.L = A.L;
+        g.n_kblk = A.n_kblk;
+        g.tiles = sc.tiles.as<uint32_t>();
+        g.n_tiles = sc.tiles_n;
+        g.triangle = run.triangle ? 1 : 0;
+        g.out_ld = run.out_ld;
+        g.out = run.d_out;
+        g.out_f64 = run.out_f64 ? 1 : 0;
+        g.flag_pairs = flags;
+        g.flag_count = count;
+        g.max_flagged = sc.max_flagged;
+        // Error model of the residual Gram: bf16 hi/lo represents e to 2^-16
+        // relative (the dropped lo*lo term is bounded by 2^-16 sqrt(g_i g_j));
+        // FP32 accumulation adds ~2^-24 sqrt(#accumulations) sqrt(g_i g_j).
+        const double n_acc = A.mode == kGramBF16x3 ? 3.0 * A.n_kblk : 8.0 * A.n_kblk;
+        g.kappa_dup = 8.0;
+        g.u_rep = A.mode == kGramBF16x3 ? std::ldexp(1.0, -16) : std::ldexp(1.0, -24);
+        g.acc_coef = 2.0 * std::ldexp(1.0, -24) * std::sqrt(n_acc);
+        g.tau = 2e-5;
+
+        if (t_main) t_main->start(s);
+        launch_gram(g, A.mode, s);
+        if (t_main) t_main->stop(s);
+        BS_HIP(hipGetLastError());
+        if (launches) ++*launches;
+        if (t_fb) t_fb->start(s);
+        launch_fallback(A.centred.as<double>(), B.centred.as<double>(), A.L, flags, count,
+                        sc.max_flagged, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, s);
+        if (t_fb) t_fb->stop(s);
+        BS_HIP(hipGetLastError());
+        BS_HIP(hipMemcpyAsync(sc.h_count, count, 4, hipMemcpyDeviceToHost, s));
+        BS_HIP(hipStreamSynchronize(s));
+        const uint32_t c = *sc.h_count;
+        if ((int)c <= sc.max_flagged) return c;
+        // capacity exceeded: grow and run again (every output is rewritten)
+        sc.max_flagged = (int)std::min<uint64_t>((uint64_t)c + c / 4, 0x7FFFFFFF / 2);
+    }
+    fail(BS_ERR_INTERNAL, "flagged-pair list overflowed twice");
+}
+
+void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m, void* d_out,
+                 bool out_f64)
+{
+    hipStream_t s = ctx->stream;
+    bs_run_stats st{};
+    st.pairs = n >= 2 ? (uint64_t)n * (n - 1) / 2 : 0;
+    if (n < 2) {
+        ctx->last = st;
+        return;
+    }
+    ctx->t_total.start(s);
+    if (!is_gram(m)) {
+        ctx->t_prep.start(s);
+        ctx->t_prep.stop(s);
+        ctx->t_main.start(s);
+        launch_plain_avsa(d_xyz, n, L, m == BS_FP64_PLAIN, d_out, out_f64, s);
+        ctx->t_main.stop(s);
+        BS_HIP(hipGetLastError());
+        ctx->t_fallback.start(s);
+        ctx->t_fallback.stop(s);
+        ctx->t_total.stop(s);
+        BS_HIP(hipStreamSynchronize(s));
+        st.main_launches = 1;
+    } else {
+        ctx->t_prep.start(s);
+        prepare(ctx->prep, d_xyz, n, L, mode_of(m), nullptr, s);
+        ctx->t_prep.stop(s);
+        GramRun run;
+        run.triangle = true;
+        run.d_out = d_out;
+        run.out_f64 = out_f64;
+        int launches = 0;
+        st.flagged_pairs = run_gram(ctx->prep, ctx->prep, run, ctx->gsc, s, &ctx->t_main,
+                                    &ctx->t_fallback, &launches);
+        ctx->t_total.stop(s);
+        BS_HIP(hipStreamSynchronize(s));
+        st.main_launches = (uint64_t)launches;
+        st.tile_m = kTileRows;
+        st.tile_n = kTileRows;
+    }
+    st.total_ms = ctx->t_total.ms();
+    st.prep_ms = ctx->t_prep.ms();
+    st.main_ms = ctx->t_main.ms();
+    st.fallback_ms = ctx->t_fallback.ms();
+    ctx->last = st;
+}
+
+}  // namespace bs

@@ -1,0 +1,89 @@
+// engine.hpp -- host orchestration shared by the ABI entry points.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "qcp.hpp"
+
+namespace bs {
+
+// Conformations of one atom count, prepared on the device for the Gram path:
+// centred FP64 copies (for FP64 re-evaluation), the rotation onto the common
+// reference, the reference itself (+ R0), operand panels and constants.
+struct Prepared {
+    int n = 0, n_pad = 0, L = 0, n_kblk = 0;
+    GramMode mode = kGramBF16x3;
+    DevBuf centred, rot, ref, partial, panels, consts;
+    void bind(Budget* b)
+    {
+        centred.bind(b); rot.bind(b); ref.bind(b); partial.bind(b); panels.bind(b); consts.bind(b);
+    }
+    void reset()
+    {
+        centred.reset(); rot.reset(); ref.reset(); partial.reset(); panels.reset(); consts.reset();
+    }
+    size_t bytes() const
+    {
+        return centred.size() + rot.size() + ref.size() + partial.size() + panels.size() +
+               consts.size();
+    }
+};
+
+// Bytes prepare() will hold for n conformations of L atoms.
+size_t prepared_bytes(int n, int L);
+
+// Centre, fit, (reference mean), pack.  If `fixed_ref` (device, 3L + 6
+// doubles, the layout of Prepared::ref) is given, conformations are fitted
+// onto it and it is copied as this set's reference; otherwise the reference is
+// one Procrustes-mean step from conformation 0.
+void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
+             const double* fixed_ref, hipStream_t s);
+
+struct GramRun {
+    bool triangle = true;
+    void* d_out = nullptr;
+    bool out_f64 = false;
+    int64_t out_ld = 0;
+};
+
+// Scratch a Gram run needs besides the prepared sets.
+struct GramScratch {
+    DevBuf flags, tiles;
+    uint64_t tiles_key = ~0ull;
+    int tiles_n = 0;
+    int max_flagged = 0;
+    uint32_t* h_count = nullptr;  // pinned
+    void bind(Budget* b) { flags.bind(b); tiles.bind(b); }
+};
+
+// Gram contraction + QCP epilogue + FP64 re-evaluation of flagged pairs over
+// rows of A x cols of B (or the upper triangle of A when triangle).  Returns
+// the number of flagged pairs.  Synchronises the stream at the end.
+uint64_t run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
+                  hipStream_t s, Timer* t_main, Timer* t_fb, int* launches);
+
+GramMode mode_of(bs_method m);
+bool is_gram(bs_method m);
+
+// All-vs-all on device-resident coordinates; fills ctx->last.
+void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m, void* d_out,
+                 bool out_f64);
+
+}  // namespace bs
+
+// The context: one device, one stream, workspaces, stats, autotune choices.
+struct bs_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bs::Budget budget{0};
+    bs::Prepared prep;         // the last ensemble's prepared set
+    bs::GramScratch gsc;
+    bs::DevBuf ws_xyz, ws_out, ws_misc;
+    bs::Timer t_total, t_prep, t_main, t_fallback;
+    bs_run_stats last{};
+    std::vector<int> tune_variant;  // autotuner: Gram tile variant per atom-count bucket
+    int n_cu = 0;
+    std::mutex mu;
+};

@@ -1,0 +1,106 @@
+// kernels.hpp -- host-side launch wrappers for the device kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace bs {
+
+// ------------------------------------------------------------ plain path
+// Per-pair kernel over the upper triangle of one ensemble: centre, 3x3
+// covariance, textbook QCP, sqrt -- in T = double (fp64 != 0) or float.
+// xyz: device, n x L x 3 f64 AoS.  out: packed triangle, f64 or f32.
+void launch_plain_avsa(const double* xyz, int n, int L, bool fp64, void* out, bool out_f64,
+                       hipStream_t s);
+
+// Query-vs-library variant of the plain kernel (dense n_q x n_lib output, f32).
+void launch_plain_qvl(const double* q_xyz, int n_q, const double* lib_xyz, int n_lib, int L,
+                      bool fp64, float* out, hipStream_t s);
+
+// ------------------------------------------------------------ Gram path
+enum GramMode : int { kGramBF16x3 = 0, kGramFP32 = 1 };
+
+// Operand panels: conformations in blocks of 32, atoms in blocks of 16.
+// One (conf block, atom block) is 6 KiB = 384 x 16 B:
+//   BF16x3: 6 planes (x_hi, x_lo, y_hi, y_lo, z_hi, z_lo) x 64 lanes x 8 bf16,
+//           lane l = conformation (l % 32), atoms 8*(l/32) .. +7 of the block;
+//   FP32:   3 planes (x, y, z) x 64 lanes x 8 f32,
+//           lane l = conformation (l % 32), atoms (l/32) + 2s, s = 0..7.
+constexpr int kConfBlock = 32;
+constexpr int kAtomBlock = 16;
+constexpr int kPanelBlockU4 = 384;  // 16-byte units per (conf block, atom block)
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+struct PrepBuffers {
+    double* centred;   // n x L x 3
+    double* rot;       // n x 9 (rotation onto the reference)
+    double* ref;       // L x 3 reference, followed by 6 doubles of R0 = sum r r^T
+    double* partial;   // scratch for the reference mean: n_slices x L x 3
+    void* panels;      // packed operands (see above)
+    double* consts;    // kNumConsts x ld, SoA
+    int ld;            // leading dimension of consts (>= n, multiple of 32)
+};
+
+// Centre every conformation (sum, divide, subtract).
+void launch_centre(const double* xyz, int n, int L, double* centred, hipStream_t s);
+// Reference fit: rotation of each centred conformation onto ref (L x 3).
+void launch_fit(const double* centred, int n, int L, const double* ref, double* rot,
+                hipStream_t s);
+// ref_out = mean over conformations of rot_i * centred_i; then R0 after it.
+void launch_mean(const double* centred, const double* rot, int n, int L, double* partial,
+                 double* ref_out, hipStream_t s);
+// Copy conformation 0 as the initial reference (plus its R0).
+void launch_ref_from_first(const double* centred, int L, double* ref, hipStream_t s);
+// Residuals e_i = rot_i x_i - ref, split into operand panels, plus constants.
+void launch_pack(const double* centred, const double* rot, const double* ref, int n, int n_pad,
+                 int L, int n_kblk, GramMode mode, void* panels, double* consts, int ld,
+                 hipStream_t s);
+
+struct GramArgs {
+    const void* panels_a;   // row operand panels
+    const void* panels_b;   // column operand panels
+    const double* consts_a; // row constants (SoA, stride lda)
+    const double* consts_b; // column constants (SoA, stride ldb)
+    int lda, ldb;
+    int n_rows, n_cols;     // valid rows / cols
+    int L;                  // real atom count (for the RMSD denominator)
+    int n_kblk;             // atom blocks of 16
+    const uint32_t* tiles;  // (tile_row << 16 | tile_col) per logical tile
+    int n_tiles;
+    int triangle;           // 1: packed upper triangle of rows==cols, only i<j
+    int64_t out_ld;         // dense output: row stride (triangle: ignored)
+    void* out;
+    int out_f64;
+    uint32_t* flag_pairs;   // (i, j) of pairs to re-evaluate in FP64
+    uint32_t* flag_count;
+    int max_flagged;
+    // flagging thresholds
+    double kappa_dup;       // H < kappa_dup * u_rep * sqrt(g_i g_j)
+    double u_rep;
+    double acc_coef;        // 2^-24 * sqrt(accumulations)
+    double tau;             // tolerated RMSD error estimate (A)
+};
+
+// Builds the logical tile order (XCD-aware super-blocks) on the host.
+// Returns the number of tiles written to `out` (capacity must be enough).
+int build_tile_order(int tiles_r, int tiles_c, bool triangle, int super, uint32_t* out);
+
+void launch_gram(const GramArgs& a, GramMode mode, hipStream_t s);
+void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
+
+// FP64 re-evaluation of flagged pairs from centred coordinates.
+void launch_fallback(const double* centred_a, const double* centred_b, int L,
+                     const uint32_t* pairs, const uint32_t* count, int max_pairs, int triangle,
+                     int n, int64_t out_ld, void* out, int out_f64, hipStream_t s);
+
+// Per-query top-k over a dense n_q x n_lib score matrix (ascending, ties by index).
+void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, float* val,
+                 hipStream_t s);
+
+// Single-pair superposition (rotation + translation) in FP64.
+void launch_superpose(const double* a, const double* b, int L, double* out13, hipStream_t s);
+
+}  // namespace bs

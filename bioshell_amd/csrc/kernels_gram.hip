@@ -1,0 +1,303 @@
+// kernels_gram.hip -- covariance as a Gram contraction on MFMA, fused with the
+// FP64 shifted-QCP epilogue, plus the FP64 re-evaluation of flagged pairs.
+//
+// For a block of pairs (i, j) the nine residual cross-covariances
+//     E_cd(i, j) = sum_k e_i,k,c * e_j,k,d          (c, d in {x, y, z})
+// are nine GEMM tiles that share three row panels (e_x, e_y, e_z of the rows)
+// and three column panels.  A wave owns a 32 x 32 block of pairs and keeps the
+// nine 32x32 FP32 accumulators of v_mfma_f32_32x32x16_bf16 (or _32x32x2f32);
+// lane l then holds column j = l % 32 and rows 8(r/4) + 4(l/32) + r%4 of all
+// nine tiles, so the 3x3 matrix of every pair it owns is lane-local and the
+// QCP epilogue runs without any exchange.
+//
+// BF16x3: operands are split e = hi + lo (two bf16) and each tile accumulates
+// hi*hi + hi*lo + lo*hi in FP32.  A work-group (4 waves) covers 64 x 64 pairs;
+// operand blocks (32 conformations x 16 atoms, 6 KiB) are double-buffered
+// through LDS.  Work-group ids are dealt to the 8 XCDs round-robin, so the
+// logical tile is taken from a contiguous per-XCD slice of a super-blocked
+// order: each XCD's L2 keeps re-using the same row and column panels.
+#include "kernels.hpp"
+#include "qcp.hpp"
+
+namespace bs {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kXcd = 8;
+
+__device__ inline int64_t tri_index(int64_t n, int64_t i, int64_t j)
+{
+    return i * n - i * (i + 1) / 2 + (j - i - 1);
+}
+
+__device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
+{
+    __shared__ uint4 lds[2][4][kPanelBlockU4];  // 2 stages x (A0, A1, B0, B1) x 6 KiB
+
+    const int wg = blockIdx.x;
+    const int per = (a.n_tiles + kXcd - 1) / kXcd;
+    const int t = (wg % kXcd) * per + (wg / kXcd);
+    if (t >= a.n_tiles) return;
+    const uint32_t code = a.tiles[t];
+    const int tr = (int)(code >> 16), tc = (int)(code & 0xFFFFu);
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+
+    const uint4* __restrict__ pa = static_cast<const uint4*>(a.panels_a);
+    const uint4* __restrict__ pb = static_cast<const uint4*>(a.panels_b);
+    const int64_t stride_cb = (int64_t)a.n_kblk * kPanelBlockU4;
+    const uint4* srcA0 = pa + (int64_t)(2 * tr) * stride_cb;
+    const uint4* srcB0 = pb + (int64_t)(2 * tc) * stride_cb;
+
+    // Six 16-byte loads per thread bring one stage (A0, A1, B0, B1 blocks) in.
+    auto src_of = [&](int u) -> const uint4* {
+        const int idx = tid + 256 * u;
+        const int blk = idx / kPanelBlockU4, off = idx - blk * kPanelBlockU4;
+        return (blk < 2 ? srcA0 + blk * stride_cb : srcB0 + (blk - 2) * stride_cb) + off;
+    };
+    const uint4* s0p = src_of(0);
+    const uint4* s1p = src_of(1);
+    const uint4* s2p = src_of(2);
+    const uint4* s3p = src_of(3);
+    const uint4* s4p = src_of(4);
+    const uint4* s5p = src_of(5);
+    uint4 st0, st1, st2, st3, st4, st5;
+#define BS_GLOAD(KB)                                          \
+    {                                                         \
+        const int64_t o_ = (int64_t)(KB) * kPanelBlockU4;     \
+        st0 = s0p[o_]; st1 = s1p[o_]; st2 = s2p[o_];          \
+        st3 = s3p[o_]; st4 = s4p[o_]; st5 = s5p[o_];          \
+    }
+#define BS_LSTORE(ST)                                         \
+    {                                                         \
+        uint4* d_ = &lds[(ST)][0][0] + tid;                   \
+        d_[0] = st0; d_[256] = st1; d_[512] = st2;            \
+        d_[768] = st3; d_[1024] = st4; d_[1280] = st5;        \
+    }
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
+
+    BS_GLOAD(0)
+    BS_LSTORE(0)
+    __syncthreads();
+    for (int kb = 0; kb < a.n_kblk; ++kb) {
+        const int cur = kb & 1;
+        const bool more = kb + 1 < a.n_kblk;
+        if (more) {
+            BS_GLOAD(kb + 1)
+        }
+        const uint4* A = lds[cur][wr];
+        const uint4* B = lds[cur][2 + wc];
+        if constexpr (MODE == kGramBF16x3) {
+            bf16x8 ah[3], al[3], bh[3], bl[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ah[c] = as_bf16x8(A[(2 * c) * 64 + lane]);
+                al[c] = as_bf16x8(A[(2 * c + 1) * 64 + lane]);
+                bh[c] = as_bf16x8(B[(2 * c) * 64 + lane]);
+                bl[c] = as_bf16x8(B[(2 * c + 1) * 64 + lane]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    acc[3 * c + d] =
+                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh[d], acc[3 * c + d], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    acc[3 * c + d] =
+                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl[d], acc[3 * c + d], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    acc[3 * c + d] =
+                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh[d], acc[3 * c + d], 0, 0, 0);
+        } else {
+            float fa[3][8], fb[3][8];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const uint4 a0 = A[c * 128 + 2 * lane], a1 = A[c * 128 + 2 * lane + 1];
+                const uint4 b0 = B[c * 128 + 2 * lane], b1 = B[c * 128 + 2 * lane + 1];
+                fa[c][0] = __uint_as_float(a0.x); fa[c][1] = __uint_as_float(a0.y);
+                fa[c][2] = __uint_as_float(a0.z); fa[c][3] = __uint_as_float(a0.w);
+                fa[c][4] = __uint_as_float(a1.x); fa[c][5] = __uint_as_float(a1.y);
+                fa[c][6] = __uint_as_float(a1.z); fa[c][7] = __uint_as_float(a1.w);
+                fb[c][0] = __uint_as_float(b0.x); fb[c][1] = __uint_as_float(b0.y);
+                fb[c][2] = __uint_as_float(b0.z); fb[c][3] = __uint_as_float(b0.w);
+                fb[c][4] = __uint_as_float(b1.x); fb[c][5] = __uint_as_float(b1.y);
+                fb[c][6] = __uint_as_float(b1.z); fb[c][7] = __uint_as_float(b1.w);
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                            fa[c][s], fb[d][s], acc[3 * c + d], 0, 0, 0);
+        }
+        if (more) {
+            BS_LSTORE(cur ^ 1)
+        }
+        __syncthreads();
+    }
+
+#undef BS_GLOAD
+#undef BS_LSTORE
+
+    // ---------------------------------------------------------------- epilogue
+    const int j = tc * 64 + wc * 32 + (lane & 31);
+    const bool jok = j < a.n_cols;
+    double cj[kNumConsts];
+#pragma unroll
+    for (int q = 0; q < kNumConsts; ++q) cj[q] = jok ? a.consts_b[(int64_t)q * a.ldb + j] : 0.0;
+    const int ibase = tr * 64 + wr * 32 + 4 * (lane >> 5);
+    const double Ld = (double)a.L;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = ibase + 8 * (r >> 2) + (r & 3);
+        const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
+        bool flag = false;
+        if (ok) {
+            double E[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) E[q] = (double)acc[q][r];
+            double ci[kNumConsts];
+#pragma unroll
+            for (int q = 0; q < kNumConsts; ++q) ci[q] = a.consts_a[(int64_t)q * a.lda + i];
+            const ShiftedOut o = qcp_shifted(E, ci, cj);
+            const double sgg = ci[10] * cj[10];
+            flag = !o.converged || (o.H < a.kappa_dup * a.u_rep * sgg) ||
+                   (a.acc_coef * sgg > a.tau * sqrt(Ld * o.rmsd2_L));
+            const double rr = o.rmsd2_L / Ld;
+            const int64_t oi = a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
+            if (a.out_f64)
+                static_cast<double*>(a.out)[oi] = sqrt(rr);
+            else
+                static_cast<float*>(a.out)[oi] = sqrtf((float)rr);
+        }
+        const uint64_t m = __ballot(flag);
+        if (m) {
+            uint32_t base = 0;
+            if (lane == __ffsll((unsigned long long)m) - 1)
+                base = atomicAdd(a.flag_count, (uint32_t)__popcll(m));
+            base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+            if (flag) {
+                const uint32_t pos = base + (uint32_t)__popcll(m & lt_mask);
+                if ((int)pos < a.max_flagged) {
+                    a.flag_pairs[2 * (int64_t)pos] = (uint32_t)i;
+                    a.flag_pairs[2 * (int64_t)pos + 1] = (uint32_t)j;
+                }
+            }
+        }
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n)
+{
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v < n) p[v] = 0u;
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// One wave per flagged pair: FP64 covariance from the centred coordinates and
+// the textbook QCP (the same arithmetic as the FP64 per-pair kernel).
+__global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ ca,
+                                                       const double* __restrict__ cbp, int L,
+                                                       const uint32_t* __restrict__ pairs,
+                                                       const uint32_t* __restrict__ count,
+                                                       int max_pairs, int triangle, int n,
+                                                       int64_t out_ld, void* __restrict__ out,
+                                                       int out_f64)
+{
+    const int lane = threadIdx.x & 63;
+    const int cnt = min((int)*count, max_pairs);
+    for (int p = blockIdx.x * 4 + (threadIdx.x >> 6); p < cnt; p += gridDim.x * 4) {
+        const int i = (int)pairs[2 * (int64_t)p], j = (int)pairs[2 * (int64_t)p + 1];
+        const double* x = ca + (int64_t)i * L * 3;
+        const double* y = cbp + (int64_t)j * L * 3;
+        double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ga = 0.0, gb = 0.0;
+        for (int k = lane; k < L; k += 64) {
+            const double a0 = x[3 * k], a1 = x[3 * k + 1], a2 = x[3 * k + 2];
+            const double b0 = y[3 * k], b1 = y[3 * k + 1], b2 = y[3 * k + 2];
+            m[0] += a0 * b0; m[1] += a0 * b1; m[2] += a0 * b2;
+            m[3] += a1 * b0; m[4] += a1 * b1; m[5] += a1 * b2;
+            m[6] += a2 * b0; m[7] += a2 * b1; m[8] += a2 * b2;
+            ga += a0 * a0 + a1 * a1 + a2 * a2;
+            gb += b0 * b0 + b1 * b1 + b2 * b2;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = wave_sum(m[q]);
+        ga = wave_sum(ga);
+        gb = wave_sum(gb);
+        if (lane == 0) {
+            const double r = qcp_rmsd<double>(m, ga, gb, L);
+            const int64_t o = triangle ? tri_index(n, i, j) : (int64_t)i * out_ld + j;
+            if (out_f64)
+                static_cast<double*>(out)[o] = r;
+            else
+                static_cast<float*>(out)[o] = (float)r;
+        }
+    }
+}
+
+}  // namespace
+
+int build_tile_order(int tiles_r, int tiles_c, bool triangle, int super, uint32_t* out)
+{
+    int n = 0;
+    const int sr_n = (tiles_r + super - 1) / super, sc_n = (tiles_c + super - 1) / super;
+    for (int sr = 0; sr < sr_n; ++sr)
+        for (int sc = triangle ? sr : 0; sc < sc_n; ++sc)
+            for (int r = sr * super; r < min((sr + 1) * super, tiles_r); ++r)
+                for (int c = sc * super; c < min((sc + 1) * super, tiles_c); ++c)
+                    if (!triangle || r <= c) out[n++] = ((uint32_t)r << 16) | (uint32_t)c;
+    return n;
+}
+
+void launch_gram(const GramArgs& a, GramMode mode, hipStream_t s)
+{
+    if (a.n_tiles <= 0) return;
+    const int per = (a.n_tiles + kXcd - 1) / kXcd;
+    const int grid = per * kXcd;
+    if (mode == kGramBF16x3)
+        gram_kernel<kGramBF16x3><<<grid, 256, 0, s>>>(a);
+    else
+        gram_kernel<kGramFP32><<<grid, 256, 0, s>>>(a);
+}
+
+void launch_zero_u32(uint32_t* p, int n, hipStream_t s)
+{
+    zero_u32_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+}
+
+void launch_fallback(const double* centred_a, const double* centred_b, int L,
+                     const uint32_t* pairs, const uint32_t* count, int max_pairs, int triangle,
+                     int n, int64_t out_ld, void* out, int out_f64, hipStream_t s)
+{
+    fallback_kernel<<<1024, 256, 0, s>>>(centred_a, centred_b, L, pairs, count, max_pairs,
+                                         triangle, n, out_ld, out, out_f64);
+}
+
+}  // namespace bs

@@ -1,0 +1,28 @@
+// testhooks.cpp -- host builds of the device QCP routines, exported for CPU
+// tests of the algebra (not part of the public ABI in include/bioshell_gpu.h).
+#include "qcp.hpp"
+
+extern "C" {
+
+// out[0] = 2 (H - mu) (= L * RMSD^2), out[1] = H, out[2] = iterations,
+// out[3] = converged (1/0).
+void bsi_qcp_shifted(const double* E, const double* ci, const double* cj, double* out)
+{
+    const bs::ShiftedOut o = bs::qcp_shifted(E, ci, cj);
+    out[0] = o.rmsd2_L;
+    out[1] = o.H;
+    out[2] = o.iterations;
+    out[3] = o.converged ? 1.0 : 0.0;
+}
+
+double bsi_qcp_rmsd_f64(const double* M, double Ga, double Gb, int n)
+{
+    return bs::qcp_rmsd<double>(M, Ga, Gb, n);
+}
+
+float bsi_qcp_rmsd_f32(const float* M, float Ga, float Gb, int n)
+{
+    return bs::qcp_rmsd<float>(M, Ga, Gb, n);
+}
+
+}  // extern "C"

@@ -1,0 +1,216 @@
+/*
+ * bioshell_gpu.h -- C ABI of the MI355X all-vs-all cRMSD engine.
+ *
+ * The reference (dgront/BioShell v4.0.0-dev) exports no C-callable symbols:
+ * bioshell-pdb and bioshell-clustering are built as rlib+cdylib
+ * (bioshell-pdb/Cargo.toml:11, bioshell-clustering/Cargo.toml:12) but carry no
+ * extern "C" items, and the crmsd function the path needs is only announced
+ * (bioshell-pdb/Cargo.toml:27, "nalgebra ... for helical axis and crmsd").
+ * Each entry point below therefore names the Rust item it stands in for:
+ * the intended bioshell_pdb::calc function, the driver template it follows,
+ * or the consumer it feeds.
+ *
+ * Conventions
+ *  - Coordinates are f64 x,y,z triples per atom (bioshell-core/src/vec3.rs:43-51),
+ *    packed AoS; an ensemble is n_conf x n_atoms x 3, row-major, the layout of
+ *    Structure::model_coordinates (bioshell-pdb/src/structure.rs:101).
+ *  - All-vs-all results are the packed upper triangle, row-major over i, i<j:
+ *    index(i,j) = i*n - i*(i+1)/2 + (j-i-1); n(n-1)/2 values.
+ *  - Every call returns a bs_status; 0 is success, negative is an error whose
+ *    text is returned by bs_last_error() (per thread).  Nothing unwinds or
+ *    aborts across this boundary: preconditions the reference asserts on
+ *    (substructures.rs:19) come back as BS_ERR_INVALID_ARGUMENT.
+ *  - Host buffers are borrowed for the duration of a synchronous call.
+ *    Device memory belongs to the context / library / server that allocated it.
+ *  - A bs_context is used by one caller at a time; distinct contexts may be
+ *    used from different threads.
+ */
+#ifndef BIOSHELL_GPU_H
+#define BIOSHELL_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t bs_status;
+
+#define BS_OK 0
+#define BS_ERR_INVALID_ARGUMENT (-1)
+#define BS_ERR_NO_DEVICE (-2)
+#define BS_ERR_HIP (-3)
+#define BS_ERR_OUT_OF_BUDGET (-4)
+#define BS_ERR_UNSUPPORTED (-5)
+#define BS_ERR_NOT_READY (-6)
+#define BS_ERR_CANCELLED (-7)
+#define BS_ERR_INTERNAL (-8)
+
+/* How the pairwise cRMSD is evaluated.
+ *  BS_FP64_PLAIN   per-pair kernel, FP64: centre, 3x3 covariance, QCP, sqrt.
+ *  BS_FP32_PLAIN   the same kernel in FP32.
+ *  BS_FP32_GRAM    covariance as a Gram contraction on FP32 MFMA, FP64 QCP epilogue.
+ *  BS_BF16X3_GRAM  covariance as a Gram contraction on BF16 MFMA with hi/lo
+ *                  split operands (hi*hi + hi*lo + lo*hi, FP32 accumulation),
+ *                  FP64 QCP epilogue and FP64 re-evaluation of flagged pairs. */
+typedef enum bs_method {
+    BS_FP64_PLAIN = 0,
+    BS_FP32_PLAIN = 1,
+    BS_FP32_GRAM = 2,
+    BS_BF16X3_GRAM = 3
+} bs_method;
+
+typedef struct bs_context bs_context;
+typedef struct bs_library bs_library;
+typedef struct bs_server bs_server;
+
+/* Timings and counters of the last all-vs-all / batch / query call on a
+ * context.  Times are device times from HIP events on the context's stream. */
+typedef struct bs_run_stats {
+    double total_ms;        /* first prep kernel start .. last kernel end      */
+    double prep_ms;         /* centring, reference fit, operand packing        */
+    double main_ms;         /* the pair kernel(s): plain or Gram+QCP            */
+    double fallback_ms;     /* FP64 re-evaluation of flagged pairs             */
+    uint64_t pairs;         /* pairs evaluated                                 */
+    uint64_t flagged_pairs; /* pairs re-evaluated in FP64                       */
+    uint64_t main_launches; /* launches of the main pair kernel                 */
+    int32_t tile_m;         /* Gram tile rows used (0 for plain methods)        */
+    int32_t tile_n;         /* Gram tile cols used                              */
+} bs_run_stats;
+
+/* ---------------------------------------------------------------- basics */
+
+/* Text of the last error raised on this thread ("" if none). */
+const char* bs_last_error(void);
+
+/* Library version string. */
+const char* bs_version(void);
+
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int bs_device_count(void);
+
+/* Creates a context on `device`.  hbm_budget_bytes caps the device memory the
+ * context (and libraries / servers made from it) may allocate; 0 = no cap. */
+bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** out);
+void bs_context_destroy(bs_context* ctx);
+bs_status bs_context_synchronize(bs_context* ctx);
+/* The hipStream_t (as void*) on which the context launches its kernels. */
+void* bs_context_stream(bs_context* ctx);
+bs_status bs_context_last_stats(const bs_context* ctx, bs_run_stats* out);
+/* Device bytes currently allocated by the context (workspaces included). */
+size_t bs_context_bytes_in_use(const bs_context* ctx);
+
+/* ------------------------------------------------------- single pair */
+
+/* cRMSD of two conformations of n_atoms each, FP64 on the GPU.
+ * Stands in for the intended `bioshell_pdb::calc::crmsd(a: &[T], b: &[T]) -> f64`
+ * (bioshell-pdb/Cargo.toml:27), generic over HasCartesians like
+ * SubstructureAxis::from_3d_points (bioshell-pdb/src/calc/substructures.rs:17). */
+bs_status bs_crmsd_pair_f64(bs_context* ctx, const double* a_xyz, const double* b_xyz,
+                            size_t n_atoms, double* out);
+
+/* Optimal superposition of b onto a: a ~= R b + t, R row-major 3x3, plus the
+ * RMSD of the fit.  Output is a bioshell_pdb::calc::Rototranslation
+ * (bioshell-pdb/src/calc/transformation.rs:22-37, applied as at 171-174). */
+bs_status bs_superpose_f64(bs_context* ctx, const double* a_xyz, const double* b_xyz,
+                           size_t n_atoms, double* rot9, double* trans3, double* rmsd);
+
+/* ------------------------------------------------------- all-vs-all */
+
+/* All-vs-all cRMSD of one ensemble; host in, host out.  out_upper holds
+ * n_conf(n_conf-1)/2 values, f64 if out_f64 != 0 else f32.
+ * Stands in for the absent structural all-vs-all driver, whose template is
+ * align_all_pairs (bioshell-seq/src/alignment/alignment_protocols.rs:83-115,
+ * triangle at 97); its result is what hierarchical_clustering reads through
+ * its Fn(usize,usize)->f32 closure (bioshell-clustering/src/hierarchical/
+ * hierarchical.rs:22-26, densified in clustering_matrix.rs:11-21). */
+bs_status bs_crmsd_all_vs_all(bs_context* ctx, const double* xyz, size_t n_conf,
+                              size_t n_atoms, bs_method method, void* out_upper, int out_f64);
+
+/* The same on device-resident buffers (d_xyz and d_out_upper are device
+ * pointers, e.g. from hipMalloc); the call is asynchronous on the context's
+ * stream.  Used when the ensemble already lives in HBM. */
+bs_status bs_crmsd_all_vs_all_device(bs_context* ctx, const double* d_xyz, size_t n_conf,
+                                     size_t n_atoms, bs_method method, void* d_out_upper,
+                                     int out_f64);
+
+/* Streams the pair space in tile x tile blocks (e.g. 4096) and copies each
+ * finished block of the packed triangle to the host while the next one is
+ * computed; out_upper is a host f32 buffer of n(n-1)/2 values. */
+bs_status bs_crmsd_all_vs_all_streamed(bs_context* ctx, const double* xyz, size_t n_conf,
+                                       size_t n_atoms, bs_method method, size_t tile,
+                                       float* out_upper);
+
+/* ------------------------------------------------------- batching */
+
+/* Many independent ensembles (uneven n_conf and n_atoms) in one persistent,
+ * graph-captured launch.  out_upper[e] receives ensemble e's packed triangle
+ * (f32).  Tile shapes come from the autotuner's cache when present. */
+bs_status bs_crmsd_batch(bs_context* ctx, const double* const* xyz, const size_t* n_conf,
+                         const size_t* n_atoms, size_t n_ens, bs_method method,
+                         float* const* out_upper);
+
+/* Runs the tile-shape autotuner for `method` over representative shapes and
+ * stores its choices in `cache_path` (JSON); loads them into the context. */
+bs_status bs_autotune(bs_context* ctx, bs_method method, const char* cache_path);
+/* Loads a previously written autotune cache into the context. */
+bs_status bs_autotune_load(bs_context* ctx, const char* cache_path);
+
+/* ------------------------------------------------------- libraries */
+
+/* A device-resident library of n_conf conformations of n_atoms each. */
+bs_status bs_library_create(bs_context* ctx, const double* xyz, size_t n_conf,
+                            size_t n_atoms, bs_library** out);
+void bs_library_destroy(bs_library* lib);
+size_t bs_library_size(const bs_library* lib);
+size_t bs_library_atoms(const bs_library* lib);
+
+/* For each of n_q query conformations, the k library members of lowest cRMSD
+ * (ascending; ties by lower index): idx[q*k + r], rmsd[q*k + r]. */
+bs_status bs_query_topk(bs_context* ctx, bs_library* lib, const double* q_xyz, size_t n_q,
+                        size_t k, uint32_t* idx, float* rmsd);
+
+/* Full query x library cRMSD matrix (f32, row-major n_q x n_lib). */
+bs_status bs_query_all(bs_context* ctx, bs_library* lib, const double* q_xyz, size_t n_q,
+                       float* out);
+
+/* ------------------------------------------------------- serving */
+
+/* A continuously batching request loop over device-resident libraries.
+ * Requests are admitted while their device memory fits in hbm_budget_bytes and
+ * a slot is free; queued queries of one library are merged into one launch. */
+bs_status bs_server_create(bs_context* ctx, bs_library* const* libs, size_t n_libs,
+                           size_t hbm_budget_bytes, int max_inflight, bs_server** out);
+void bs_server_destroy(bs_server* srv);
+/* Submits n_q queries against library lib_index; q_xyz must stay valid until
+ * the ticket completes.  Results go to idx/rmsd (n_q*k each). */
+bs_status bs_server_submit(bs_server* srv, size_t lib_index, const double* q_xyz, size_t n_q,
+                           size_t k, uint32_t* idx, float* rmsd, uint64_t* ticket);
+/* 0 = done, BS_ERR_NOT_READY = still running, other = the request's error. */
+bs_status bs_server_poll(bs_server* srv, uint64_t ticket);
+/* Blocks until the ticket completes; returns its status. */
+bs_status bs_server_wait(bs_server* srv, uint64_t ticket);
+/* Cancels a ticket that has not been admitted yet. */
+bs_status bs_server_cancel(bs_server* srv, uint64_t ticket);
+/* Latency of a completed ticket (submit .. result visible on host), in ms. */
+bs_status bs_server_latency_ms(bs_server* srv, uint64_t ticket, double* ms);
+
+/* ------------------------------------------------------- synthetic input */
+
+/* Deterministic synthetic ensemble: a random-walk CA chain placed by NeRF
+ * (bond 3.8 A, bond angle U(85,150) deg, dihedral U(-180,180] deg; the
+ * PI - planar convention of bioshell-builder/src/nerf.rs:6-30), then per
+ * conformation iid N(0, s^2) noise with s ~ U(sigma_lo, sigma_hi) (or
+ * s = near_sigma with probability near_fraction), a uniform random rotation
+ * and a translation U(-50,50)^3.  out_xyz: n_conf x n_atoms x 3.
+ * Conformation i depends only on (seed, i), so any subset can be regenerated. */
+bs_status bs_generate_ensemble(uint64_t seed, size_t n_conf, size_t n_atoms, double sigma_lo,
+                               double sigma_hi, double near_fraction, double near_sigma,
+                               double* out_xyz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIOSHELL_GPU_H */

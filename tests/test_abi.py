@@ -1,0 +1,171 @@
+"""CPU-side checks of the C ABI: symbols, errors, the generator, host validation.
+
+No compute call here touches a GPU.
+"""
+import ctypes
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+import bioshell_amd as B
+from bioshell_amd import _native
+from tests.conftest import ROOT
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "bioshell_gpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    syms = header_symbols()
+    assert len(syms) >= 30
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # and the Python mirror declares exactly these
+    assert sorted(_native.SIGNATURES) == syms
+
+
+def test_version_and_device_count():
+    assert "gfx950" in B.version()
+    assert B.device_count() >= 0
+
+
+@pytest.mark.skipif(B.device_count() > 0, reason="checks the no-device error path")
+def test_context_without_device_is_an_error():
+    with pytest.raises(B.BioshellGpuError) as e:
+        B.Context(0)
+    assert e.value.status == -2
+    assert "no HIP device" in str(e.value)
+
+
+def test_null_and_invalid_arguments_return_status():
+    L = _native.lib()
+    assert L.bs_context_create(0, 0, None) == -1
+    assert b"null" in L.bs_last_error()
+    assert L.bs_crmsd_pair_f64(None, None, None, 3, None) == -1
+    assert b"context is null" in L.bs_last_error()
+    assert L.bs_context_last_stats(None, None) == -1
+    out = np.empty((2, 3, 3))
+    assert L.bs_generate_ensemble(1, 2, 3, 2.0, 1.0, 0.0, 0.0, out.ctypes.data_as(_native.c_dp)) == -1
+    assert b"sigma" in L.bs_last_error()
+    # error state is per call: a good call clears it
+    assert L.bs_generate_ensemble(1, 2, 3, 1.0, 1.0, 0.0, 0.0, out.ctypes.data_as(_native.c_dp)) == 0
+    assert L.bs_last_error() == b""
+
+
+def test_python_shape_validation():
+    with pytest.raises(ValueError):
+        B.crmsd(np.zeros((4, 3)), np.zeros((5, 3)))
+    with pytest.raises(ValueError):
+        B.crmsd_all_vs_all(np.zeros((4, 3)))
+
+
+# ------------------------------------------------------------ generator
+def test_generator_deterministic_and_subset_stable():
+    a = B.generate_ensemble(42, 30, 50, sigma=(0.5, 4.0), near_fraction=0.2)
+    b = B.generate_ensemble(42, 30, 50, sigma=(0.5, 4.0), near_fraction=0.2)
+    assert np.array_equal(a, b)
+    c = B.generate_ensemble(42, 10, 50, sigma=(0.5, 4.0), near_fraction=0.2)
+    assert np.array_equal(a[:10], c)  # conformation i depends only on (seed, i)
+    d = B.generate_ensemble(43, 10, 50, sigma=(0.5, 4.0), near_fraction=0.2)
+    assert not np.array_equal(c, d)
+
+
+def test_generator_chain_geometry():
+    """sigma = 0: every conformation is a rigid copy of the NeRF chain."""
+    x = B.generate_ensemble(5, 3, 200, sigma=0.0)
+    for c in x:
+        bonds = np.linalg.norm(np.diff(c, axis=0), axis=1)
+        assert np.allclose(bonds, 3.8, atol=1e-9)
+        u = c[:-2] - c[1:-1]
+        v = c[2:] - c[1:-1]
+        ang = np.degrees(np.arccos((u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1)))
+        assert ang.min() >= 85 - 1e-6 and ang.max() <= 150 + 1e-6
+    # rigid copies: identical inter-atomic distances
+    d0 = np.linalg.norm(x[0][:, None] - x[0][None], axis=2)
+    d1 = np.linalg.norm(x[1][:, None] - x[1][None], axis=2)
+    assert np.abs(d0 - d1).max() < 1e-9
+    # translations within the +-50 A box (centroid of a rotated centred chain + t)
+    cen = x.mean(axis=1)
+    assert np.abs(cen).max() < 50 + np.abs(x[0] - x[0].mean(0)).max()
+
+
+def test_generator_noise_levels():
+    x = B.generate_ensemble(9, 400, 100, sigma=(2.0, 2.0), near_fraction=0.25, near_sigma=0.01)
+    base = B.generate_ensemble(9, 1, 100, sigma=0.0, near_fraction=0.0)[0]
+    from oracle import oracle as O
+
+    r = np.array([O.crmsd(c, base) for c in x])
+    near = r < 0.1
+    assert 0.15 < near.mean() < 0.35
+    assert np.all(r[near] < 0.05)
+    # iid N(0, s^2) per coordinate: RMSD to base ~ s sqrt(3) (slightly less after fitting)
+    assert np.median(r[~near]) == pytest.approx(2.0 * np.sqrt(3), rel=0.1)
+
+
+def test_generator_golden(golden_config0):
+    x = B.generate_ensemble(0, 200, 76, sigma=1.5)
+    assert np.array_equal(x[:2, :4], golden_config0["first_atoms"])
+    assert hashlib.sha256(x.tobytes()).digest() == golden_config0["sha256"].tobytes()
+
+
+# ------------------------------------------------------------ host QCP builds
+def _hook(name, res, args):
+    fn = getattr(_native.lib(), name)
+    fn.restype = res
+    fn.argtypes = args
+    return fn
+
+
+def test_host_build_of_device_qcp_matches_oracle():
+    from oracle import oracle as O
+
+    dp = ctypes.POINTER(ctypes.c_double)
+    f64 = _hook("bsi_qcp_rmsd_f64", ctypes.c_double, [dp, ctypes.c_double, ctypes.c_double, ctypes.c_int])
+    rng = np.random.default_rng(0)
+    for L in (2, 3, 7, 76):
+        for _ in range(5):
+            a = rng.normal(size=(L, 3)) * 5
+            b = a + rng.normal(size=(L, 3))
+            ac, bc = O.centre(a)[0], O.centre(b)[0]
+            M, ga, gb = O.covariance(ac, bc)
+            M = np.ascontiguousarray(M.ravel())
+            assert f64(M.ctypes.data_as(dp), ga, gb, L) == pytest.approx(O.crmsd(a, b), abs=1e-9)
+
+
+def test_host_build_of_shifted_qcp_matches_oracle():
+    """The shifted-frame solve (residual Gram + per-conformation constants)."""
+    from oracle import oracle as O
+
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn = _hook("bsi_qcp_shifted", None, [dp, dp, dp, dp])
+
+    def consts(e, r):
+        C, R0, g = r.T @ e, r.T @ r, (e * e).sum()
+        t0, tc = np.trace(R0), np.trace(C)
+        return np.array([
+            R0[0, 0] - t0 + 2 * (C[0, 0] - tc), R0[1, 1] - t0 + 2 * (C[1, 1] - tc),
+            R0[2, 2] - t0 + 2 * (C[2, 2] - tc), R0[0, 1] + C[0, 1] + C[1, 0],
+            R0[0, 2] + C[0, 2] + C[2, 0], R0[1, 2] + C[1, 2] + C[2, 1],
+            C[1, 2] - C[2, 1], C[2, 0] - C[0, 2], C[0, 1] - C[1, 0], g / 2, np.sqrt(g)])
+
+    x = B.generate_ensemble(3, 24, 50, sigma=(0.5, 4.0), near_fraction=0.2)
+    xc = np.array([O.centre(a)[0] for a in x])
+    # any reference works (the algebra is exact); use a noisy one, unaligned
+    ref = xc[0] + np.random.default_rng(1).normal(size=xc[0].shape)
+    ref -= ref.mean(0)
+    e = xc - ref
+    cs = [consts(ei, ref) for ei in e]
+    out = np.empty(4)
+    for i in range(24):
+        for j in range(i + 1, 24):
+            E = np.ascontiguousarray((e[i].T @ e[j]).ravel())
+            fn(E.ctypes.data_as(dp), cs[i].ctypes.data_as(dp), cs[j].ctypes.data_as(dp), out.ctypes.data_as(dp))
+            assert out[3] == 1.0
+            assert np.sqrt(out[0] / 50) == pytest.approx(O.crmsd(x[i], x[j]), abs=1e-8)

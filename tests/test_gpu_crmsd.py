@@ -1,0 +1,176 @@
+"""GPU parity of the all-vs-all cRMSD path against the FP64 oracle, through the C ABI.
+
+Tolerances (|dRMSD| in A):
+  FP64 plain      1e-5   (the FP64 gate)
+  FP32 plain      5e-3   (reported; textbook QCP in FP32 cancels E0 - lambda)
+  FP32 Gram       1e-4
+  BF16x3 Gram     1e-4   (the split-BF16 target)
+"""
+import numpy as np
+import pytest
+
+import bioshell_amd as B
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {
+    B.Method.FP64_PLAIN: 1e-5,
+    B.Method.FP32_PLAIN: 5e-3,
+    B.Method.FP32_GRAM: 1e-4,
+    B.Method.BF16X3_GRAM: 1e-4,
+}
+ALL = list(TOL)
+
+
+def run(ctx, x, method, out_f64=None):
+    if out_f64 is None:
+        out_f64 = method == B.Method.FP64_PLAIN
+    return B.crmsd_all_vs_all(x, method, out_f64=out_f64, ctx=ctx).upper.astype(np.float64)
+
+
+def report(name, err):
+    print(f"{name}: max |dRMSD| = {err.max():.3e}  mean = {err.mean():.3e}", flush=True)
+
+
+# ---------------------------------------------------------------- single pair
+def test_pair_fp64(gpu_ctx):
+    rng = np.random.default_rng(0)
+    for L in (1, 2, 3, 17, 76, 300):
+        a = rng.normal(size=(L, 3)) * 6
+        b = a + rng.normal(size=(L, 3))
+        assert B.crmsd(a, b, ctx=gpu_ctx) == pytest.approx(O.crmsd(a, b), abs=1e-9)
+
+
+def test_superpose_fp64(gpu_ctx):
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=(40, 3)) * 6
+    b = (a + rng.normal(size=a.shape) * 0.5) @ np.linalg.qr(rng.normal(size=(3, 3)))[0].T + 20.0
+    if np.linalg.det(np.linalg.qr(rng.normal(size=(3, 3)))[0]) < 0:
+        pass
+    T, r = B.superpose(a, b, ctx=gpu_ctx)
+    R, t, r0 = O.superpose(a, b)
+    assert r == pytest.approx(r0, abs=1e-9)
+    assert np.allclose(T.rotation, R, atol=1e-9)
+    assert np.allclose(T.translation, t, atol=1e-8)
+    assert np.sqrt(((a - T.apply(b)) ** 2).sum(1).mean()) == pytest.approx(r0, abs=1e-9)
+
+
+# ---------------------------------------------------------------- golden
+@pytest.mark.parametrize("method", ALL)
+def test_golden_2jqb(gpu_ctx, golden_2jqb, method):
+    for key in ("ca", "all_atom"):
+        got = run(gpu_ctx, golden_2jqb[key], method)
+        err = np.abs(got - golden_2jqb[key + "_rmsd"])
+        report(f"2jqb {key} {method.name}", err)
+        assert err.max() <= TOL[method]
+
+
+@pytest.mark.parametrize("method", ALL)
+def test_config0_all_vs_all(gpu_ctx, golden_config0, method):
+    """200 conformations x 76 CA, sigma 1.5 A: 19,900 pairs vs the committed golden values."""
+    x = B.generate_ensemble(0, 200, 76, sigma=1.5)
+    got = run(gpu_ctx, x, method)
+    err = np.abs(got - golden_config0["rmsd"])
+    report(f"config0 {method.name}", err)
+    assert err.max() <= TOL[method]
+    st = gpu_ctx.last_stats()
+    assert st["pairs"] == 19900
+
+
+# ---------------------------------------------------------------- ragged shapes
+@pytest.mark.parametrize("n", [2, 3, 31, 32, 33, 63, 64, 65, 130])
+@pytest.mark.parametrize("L", [1, 2, 3, 15, 16, 17, 33])
+def test_ragged_shapes(gpu_ctx, n, L):
+    x = B.generate_ensemble(100 + n * 7 + L, n, L, sigma=(0.5, 3.0), near_fraction=0.1)
+    want = O.all_vs_all(x, nthreads=4)
+    for method in ALL:
+        got = run(gpu_ctx, x, method)
+        assert got.shape == want.shape
+        # L = 1, 2 are (near-)degenerate: every method must still agree
+        err = np.abs(got - want)
+        assert err.max() <= max(TOL[method], 2e-5 if L <= 2 else 0), (method.name, err.max())
+
+
+def test_single_conformation_and_errors(gpu_ctx):
+    x = B.generate_ensemble(1, 1, 10)
+    assert B.crmsd_all_vs_all(x, ctx=gpu_ctx).upper.size == 0
+    with pytest.raises(B.BioshellGpuError) as e:
+        B.crmsd_all_vs_all(np.zeros((3, 0, 3)), ctx=gpu_ctx)
+    assert e.value.status == -1
+
+
+# ---------------------------------------------------------------- precision
+def test_config1_shape_gram_vs_plain(gpu_ctx):
+    """2,000 x 150 CA, sigma U(0.5, 4): FP32 plain vs FP32-MFMA Gram, both vs FP64."""
+    x = B.generate_ensemble(1, 2000, 150, sigma=(0.5, 4.0))
+    want = O.all_vs_all(x, nthreads=8)
+    for method in (B.Method.FP32_PLAIN, B.Method.FP32_GRAM, B.Method.BF16X3_GRAM):
+        err = np.abs(run(gpu_ctx, x, method) - want)
+        report(f"config1 {method.name}", err)
+        assert err.max() <= TOL[method]
+
+
+def test_near_identical_pairs(gpu_ctx):
+    """5 % near-identical conformations (sigma 0.01 A) stress degenerate QCP roots."""
+    x = B.generate_ensemble(2, 600, 300, sigma=(0.5, 4.0), near_fraction=0.05, near_sigma=0.01)
+    want = O.all_vs_all(x, nthreads=8)
+    for method in (B.Method.FP64_PLAIN, B.Method.FP32_GRAM, B.Method.BF16X3_GRAM):
+        err = np.abs(run(gpu_ctx, x, method) - want)
+        report(f"near-identical {method.name}", err)
+        assert err.max() <= TOL[method]
+        small = want < 0.1
+        assert small.sum() > 100
+        assert err[small].max() <= TOL[method]
+
+
+def test_duplicates_far_from_reference(gpu_ctx):
+    """Exact and near duplicates of strongly perturbed conformations: the split
+    products' error is largest here, so these pairs must be re-evaluated."""
+    base = B.generate_ensemble(4, 40, 120, sigma=(3.0, 4.0))
+    rng = np.random.default_rng(4)
+    dup = base + rng.normal(size=base.shape) * 1e-3
+    x = np.concatenate([base, base, dup])
+    want = O.all_vs_all(x, nthreads=8)
+    for method in (B.Method.FP32_GRAM, B.Method.BF16X3_GRAM):
+        err = np.abs(run(gpu_ctx, x, method) - want)
+        report(f"duplicates {method.name}", err)
+        assert err.max() <= TOL[method]
+        assert gpu_ctx.last_stats()["flagged_pairs"] >= 40
+
+
+def test_order_and_rigid_invariance(gpu_ctx):
+    x = B.generate_ensemble(5, 90, 64, sigma=(0.5, 3.0))
+    n = len(x)
+    d0 = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx).to_dense()
+    perm = np.random.default_rng(5).permutation(n)
+    d1 = B.crmsd_all_vs_all(x[perm], B.Method.BF16X3_GRAM, ctx=gpu_ctx).to_dense()
+    assert np.abs(d1 - d0[np.ix_(perm, perm)]).max() <= 2e-4
+    y = x.copy()
+    Q = np.linalg.qr(np.random.default_rng(6).normal(size=(3, 3)))[0]
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] *= -1
+    y[7] = y[7] @ Q.T + 30.0
+    d2 = B.crmsd_all_vs_all(y, B.Method.BF16X3_GRAM, ctx=gpu_ctx).to_dense()
+    assert np.abs(d2 - d0).max() <= 2e-4
+
+
+def test_budget_is_enforced():
+    with B.Context(0, hbm_budget_bytes=1 << 16) as ctx:
+        x = B.generate_ensemble(6, 300, 100)
+        with pytest.raises(B.BioshellGpuError) as e:
+            B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=ctx)
+        assert e.value.status == -4
+
+
+def test_device_pointer_entry(gpu_ctx):
+    import torch
+
+    x = B.generate_ensemble(7, 257, 90, sigma=(0.5, 3.0), near_fraction=0.05)
+    want = O.all_vs_all(x, nthreads=8)
+    dx = torch.from_numpy(x).to("cuda:0")
+    dout = torch.empty(len(want), dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    B.crmsd_all_vs_all_device(dx.data_ptr(), 257, 90, dout.data_ptr(), B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    got = dout.cpu().numpy().astype(np.float64)
+    assert np.abs(got - want).max() <= 1e-4
