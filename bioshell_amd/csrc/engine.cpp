@@ -16,7 +16,7 @@ size_t prepared_bytes(int n, int L)
 {
     const size_t n_pad = (size_t)round_up(n, kTileRows);
     const size_t n_kblk = (size_t)(L + kAtomBlock - 1) / kAtomBlock;
-    const size_t slices = (size_t)(n + 63) / 64;
+    const size_t slices = (size_t)(n + kMeanSliceConfs - 1) / kMeanSliceConfs;
     return (size_t)n * L * 3 * 8 + (size_t)n * 9 * 8 + (size_t)(3 * L + 6) * 8 +
            slices * 3 * L * 8 + n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16 +
            (size_t)kNumConsts * n_pad * 8;
@@ -30,7 +30,7 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
     P.n_pad = round_up(n, kTileRows);
     P.n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
     P.mode = mode;
-    const int slices = (n + 63) / 64;
+    const int slices = (n + kMeanSliceConfs - 1) / kMeanSliceConfs;
     double* centred = static_cast<double*>(P.centred.ensure((size_t)n * L * 3 * 8, "centred coordinates"));
     double* rot = static_cast<double*>(P.rot.ensure((size_t)n * 9 * 8, "rotations"));
     double* ref = static_cast<double*>(P.ref.ensure((size_t)(3 * L + 6) * 8, "reference"));
@@ -107,8 +107,22 @@ uint64_t run_gram(const Prepared& A, const Prepared& B, const GramRun& run, Gram
         const double n_acc = A.mode == kGramBF16x3 ? 3.0 * A.n_kblk : 8.0 * A.n_kblk;
         g.kappa_dup = 8.0;
         g.u_rep = A.mode == kGramBF16x3 ? std::ldexp(1.0, -16) : std::ldexp(1.0, -24);
+        // Error of H - mu in units of sqrt(g_i g_j): FP32 accumulation
+        // (~2^-24 per accumulation, random-walk) plus, for the split, the
+        // dropped lo*lo products (|sum lo_i lo_j| <= 2^-16 sqrt(g_i g_j), a sum
+        // of 3L terms of random sign: 3 sigma = 3/sqrt(3L) of that bound).
         g.acc_coef = 2.0 * std::ldexp(1.0, -24) * std::sqrt(n_acc);
+        if (A.mode == kGramBF16x3)
+            g.acc_coef += g.u_rep * std::min(1.0, 3.0 / std::sqrt(3.0 * A.L));
         g.tau = 2e-5;
+        // Rounding e to the split representation moves each coordinate by up
+        // to u_rep |e|; its effect on RMSD has a standard deviation of about
+        // u_rep sqrt(2 (g_i + g_j)) / (3 L).  Pairs whose 6-sigma value could
+        // exceed 5e-5 A (tiny L, large residuals) are re-evaluated in FP64.
+        {
+            const double t = 5e-5 * 3.0 * A.L / (6.0 * std::sqrt(2.0) * g.u_rep);
+            g.rep_g_max = t * t;
+        }
 
         if (t_main) t_main->start(s);
         launch_gram(g, A.mode, s);

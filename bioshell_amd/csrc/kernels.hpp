@@ -32,17 +32,9 @@ constexpr int kConfBlock = 32;
 constexpr int kAtomBlock = 16;
 constexpr int kPanelBlockU4 = 384;  // 16-byte units per (conf block, atom block)
 
-inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+constexpr int kMeanSliceConfs = 32;  // conformations per partial sum of the reference mean
 
-struct PrepBuffers {
-    double* centred;   // n x L x 3
-    double* rot;       // n x 9 (rotation onto the reference)
-    double* ref;       // L x 3 reference, followed by 6 doubles of R0 = sum r r^T
-    double* partial;   // scratch for the reference mean: n_slices x L x 3
-    void* panels;      // packed operands (see above)
-    double* consts;    // kNumConsts x ld, SoA
-    int ld;            // leading dimension of consts (>= n, multiple of 32)
-};
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // Centre every conformation (sum, divide, subtract).
 void launch_centre(const double* xyz, int n, int L, double* centred, hipStream_t s);
@@ -82,6 +74,7 @@ struct GramArgs {
     double u_rep;
     double acc_coef;        // 2^-24 * sqrt(accumulations)
     double tau;             // tolerated RMSD error estimate (A)
+    double rep_g_max;       // g_i + g_j above which the split's rounding of e may show
 };
 
 // Builds the logical tile order (XCD-aware super-blocks) on the host.

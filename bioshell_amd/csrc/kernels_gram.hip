@@ -159,53 +159,103 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
 #undef BS_LSTORE
 
     // ---------------------------------------------------------------- epilogue
-    const int j = tc * 64 + wc * 32 + (lane & 31);
+    // The LDS stage buffers are free now.  Layout (bytes):
+    //   [0, 12288)       constants of the tile's 64 rows and 64 columns, 12 doubles each
+    //   [12288, 49152)   per wave, 4 accumulator slots x 9 values x 64 lanes (f32)
+    // Accumulators go through LDS in 4 rounds of 4 slots so that one copy of
+    // the FP64 solve serves all 16 pairs a lane owns.
+    double* sconst = reinterpret_cast<double*>(&lds[0][0][0]);
+    float* sacc = reinterpret_cast<float*>(reinterpret_cast<char*>(&lds[0][0][0]) + 12288) +
+                  wv * (4 * 9 * 64);
+    for (int v = tid; v < kNumConsts * 128; v += 256) {
+        const int q = v / 128, r = v % 128;
+        const bool is_row = r < 64;
+        const int conf = is_row ? tr * 64 + r : tc * 64 + (r - 64);
+        const double val = is_row ? (conf < a.n_rows ? a.consts_a[(int64_t)q * a.lda + conf] : 0.0)
+                                  : (conf < a.n_cols ? a.consts_b[(int64_t)q * a.ldb + conf] : 0.0);
+        sconst[r * 12 + q] = val;
+    }
+    __syncthreads();
+
+    const int jl = wc * 32 + (lane & 31);  // column within the tile
+    const int j = tc * 64 + jl;
     const bool jok = j < a.n_cols;
     double cj[kNumConsts];
 #pragma unroll
-    for (int q = 0; q < kNumConsts; ++q) cj[q] = jok ? a.consts_b[(int64_t)q * a.ldb + j] : 0.0;
-    const int ibase = tr * 64 + wr * 32 + 4 * (lane >> 5);
+    for (int q = 0; q < kNumConsts; ++q) cj[q] = sconst[(64 + jl) * 12 + q];
+    const int il0 = wr * 32 + 4 * (lane >> 5);  // first row within the tile of this lane
     const double Ld = (double)a.L;
+    const double invL = 1.0 / Ld;
+    const double tau2L = a.tau * a.tau * Ld;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
+#define BS_STASH(BASE)                                                   \
+    _Pragma("unroll") for (int s_ = 0; s_ < 4; ++s_)                     \
+        _Pragma("unroll") for (int q_ = 0; q_ < 9; ++q_)                 \
+            sacc[(s_ * 9 + q_) * 64 + lane] = acc[q_][(BASE) + s_];
+
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = ibase + 8 * (r >> 2) + (r & 3);
-        const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
-        bool flag = false;
-        if (ok) {
-            double E[9];
+    for (int round = 0; round < 4; ++round) {
+        BS_STASH(4 * round)
+#pragma unroll 1
+        for (int sl = 0; sl < 4; sl += 2) {
+            // accumulator slot r = 4 round + sl holds row 8 (r/4) + r%4 of the
+            // lane's block; slots sl and sl + 1 are solved together
+            const int il = il0 + 8 * round + sl;
+            const int i0 = tr * 64 + il, i1 = i0 + 1;
+            double E0[9], E1[9], c0[kNumConsts], c1[kNumConsts];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) E[q] = (double)acc[q][r];
-            double ci[kNumConsts];
+            for (int q = 0; q < 9; ++q) {
+                E0[q] = (double)sacc[(sl * 9 + q) * 64 + lane];
+                E1[q] = (double)sacc[((sl + 1) * 9 + q) * 64 + lane];
+            }
 #pragma unroll
-            for (int q = 0; q < kNumConsts; ++q) ci[q] = a.consts_a[(int64_t)q * a.lda + i];
-            const ShiftedOut o = qcp_shifted(E, ci, cj);
-            const double sgg = ci[10] * cj[10];
-            flag = !o.converged || (o.H < a.kappa_dup * a.u_rep * sgg) ||
-                   (a.acc_coef * sgg > a.tau * sqrt(Ld * o.rmsd2_L));
-            const double rr = o.rmsd2_L / Ld;
-            const int64_t oi = a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
-            if (a.out_f64)
-                static_cast<double*>(a.out)[oi] = sqrt(rr);
-            else
-                static_cast<float*>(a.out)[oi] = sqrtf((float)rr);
-        }
-        const uint64_t m = __ballot(flag);
-        if (m) {
-            uint32_t base = 0;
-            if (lane == __ffsll((unsigned long long)m) - 1)
-                base = atomicAdd(a.flag_count, (uint32_t)__popcll(m));
-            base = __shfl(base, __ffsll((unsigned long long)m) - 1);
-            if (flag) {
-                const uint32_t pos = base + (uint32_t)__popcll(m & lt_mask);
-                if ((int)pos < a.max_flagged) {
-                    a.flag_pairs[2 * (int64_t)pos] = (uint32_t)i;
-                    a.flag_pairs[2 * (int64_t)pos + 1] = (uint32_t)j;
+            for (int q = 0; q < kNumConsts; ++q) {
+                c0[q] = sconst[il * 12 + q];
+                c1[q] = sconst[(il + 1) * 12 + q];
+            }
+            ShiftedOut o[2];
+            qcp_shifted2(E0, c0, E1, c1, cj, o[0], o[1]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = h ? i1 : i0;
+                const double* ci = h ? c1 : c0;
+                const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
+                bool flag = false;
+                if (ok) {
+                    const double sgg = ci[10] * cj[10];
+                    const double err = a.acc_coef * sgg;
+                    // flag: unconverged / near-double root, near-duplicate
+                    // residuals, estimated error above tau (compared squared:
+                    // no FP64 sqrt), or residuals too large for the split
+                    flag = !o[h].converged || (o[h].H < a.kappa_dup * a.u_rep * sgg) ||
+                           (err * err > tau2L * o[h].rmsd2_L) ||
+                           (2.0 * (ci[9] + cj[9]) > a.rep_g_max);
+                    const int64_t oi =
+                        a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
+                    if (a.out_f64)
+                        static_cast<double*>(a.out)[oi] = sqrt(o[h].rmsd2_L * invL);
+                    else
+                        static_cast<float*>(a.out)[oi] = sqrtf((float)(o[h].rmsd2_L * invL));
+                }
+                const uint64_t m = __ballot(flag);
+                if (m) {
+                    const int leader = __ffsll((unsigned long long)m) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(a.flag_count, (uint32_t)__popcll(m));
+                    base = __shfl(base, leader);
+                    if (flag) {
+                        const uint32_t pos = base + (uint32_t)__popcll(m & lt_mask);
+                        if ((int)pos < a.max_flagged) {
+                            a.flag_pairs[2 * (int64_t)pos] = (uint32_t)i;
+                            a.flag_pairs[2 * (int64_t)pos + 1] = (uint32_t)j;
+                        }
+                    }
                 }
             }
         }
     }
+#undef BS_STASH
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n)

@@ -70,9 +70,12 @@ __device__ void rotation_from_cov(const double* M, double* R, double* lambda)
 }
 
 // ------------------------------------------------------------------- fit
-__global__ __launch_bounds__(256) void fit_kernel(const double* __restrict__ centred, int n, int L,
-                                                  const double* __restrict__ ref,
-                                                  double* __restrict__ rot)
+// Two stages: one wave per conformation reduces M = sum x_k r_k^T into
+// cov (9 doubles per conformation); then one thread per conformation solves
+// Horn's 4x4 eigenproblem and overwrites its 9 doubles with the rotation.
+__global__ __launch_bounds__(256) void fit_cov_kernel(const double* __restrict__ centred, int n,
+                                                      int L, const double* __restrict__ ref,
+                                                      double* __restrict__ cov)
 {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -88,16 +91,32 @@ __global__ __launch_bounds__(256) void fit_kernel(const double* __restrict__ cen
     }
 #pragma unroll
     for (int q = 0; q < 9; ++q) m[q] = wave_sum(m[q]);
-    if (lane == 0) {
-        double R[9];
-        rotation_from_cov(m, R, nullptr);
+    if (lane < 9) {
+        double v = m[0];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) rot[(int64_t)i * 9 + q] = R[q];
+        for (int q = 1; q < 9; ++q)
+            if (lane == q) v = m[q];
+        cov[(int64_t)i * 9 + lane] = v;
     }
 }
 
+__global__ __launch_bounds__(64) void fit_rot_kernel(double* __restrict__ rot, int n)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    double m[9], R[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = rot[(int64_t)i * 9 + q];
+    rotation_from_cov(m, R, nullptr);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) rot[(int64_t)i * 9 + q] = R[q];
+}
+
 // ------------------------------------------------------------------ mean
-constexpr int kMeanSlice = 64;
+// Stage 1: slices of 32 conformations, each thread one coordinate of the
+// rotated sum.  Stage 2: one wave per coordinate adds the slices in a fixed
+// order (deterministic, no atomics).
+constexpr int kMeanSlice = kMeanSliceConfs;
 
 __global__ __launch_bounds__(256) void mean_partial_kernel(const double* __restrict__ centred,
                                                            const double* __restrict__ rot, int n,
@@ -124,11 +143,13 @@ __global__ __launch_bounds__(256) void mean_final_kernel(const double* __restric
                                                          int n_slices, int n, int L,
                                                          double* __restrict__ ref)
 {
-    const int v = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (v >= 3 * L) return;
     double acc = 0.0;
-    for (int s = 0; s < n_slices; ++s) acc += partial[(int64_t)s * 3 * L + v];
-    ref[v] = acc / n;
+    for (int s = lane; s < n_slices; s += 64) acc += partial[(int64_t)s * 3 * L + v];
+    acc = wave_sum(acc);
+    if (lane == 0) ref[v] = acc / n;
 }
 
 // R0 = sum_k r_k r_k^T, stored as (xx, yy, zz, xy, xz, yz) after the 3L values.
@@ -171,6 +192,13 @@ __device__ inline double bf16_to_double(uint16_t h)
     return (double)__uint_as_float(((uint32_t)h) << 16);
 }
 
+// One wave per conformation, atoms in chunks of kPackChunk atom blocks.
+// Phase 1: lanes walk consecutive atoms (coalesced reads), form e = U x - r,
+// split it, accumulate g and C, and drop each operand value into an LDS image
+// of the panel words.  Phase 2: lanes copy those words out as 16-byte stores.
+constexpr int kPackChunk = 16;                 // atom blocks per chunk (256 atoms)
+constexpr int kPackWordsU4 = kPackChunk * 12;  // 16-byte words per chunk and conformation
+
 template <int MODE>
 __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ centred,
                                                    const double* __restrict__ rot,
@@ -179,26 +207,26 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ ce
                                                    uint4* __restrict__ panels,
                                                    double* __restrict__ consts, int ld)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n_pad) return;
+    __shared__ uint4 img[4][kPackWordsU4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wv;  // grid covers n_pad exactly
     const bool live = i < n;
     double R[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) R[q] = live ? rot[(int64_t)i * 9 + q] : 0.0;
     const double* x = centred + (int64_t)(live ? i : 0) * L * 3;
     const int cb = i / kConfBlock, crow = i % kConfBlock;
+    uint16_t* img16 = reinterpret_cast<uint16_t*>(img[wv]);
+    float* img32 = reinterpret_cast<float*>(img[wv]);
 
     double g = 0.0;
     double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // sum r e~^T
-    for (int grp = lane; grp < 2 * n_kblk; grp += 64) {
-        const int kb = grp >> 1, h = grp & 1;
-        uint32_t words[6][4];  // 6 uint4 outputs
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int k = kb * kAtomBlock + (MODE == kGramBF16x3 ? 8 * h + e : h + 2 * e);
-            double ev[3] = {0.0, 0.0, 0.0};
-            double rv[3] = {0.0, 0.0, 0.0};
+    for (int kb0 = 0; kb0 < n_kblk; kb0 += kPackChunk) {
+        const int nkb = min(kPackChunk, n_kblk - kb0);
+        for (int kk = lane; kk < nkb * kAtomBlock; kk += 64) {
+            const int k = kb0 * kAtomBlock + kk;
+            const int kbl = kk / kAtomBlock, w = kk % kAtomBlock;
+            double ev[3] = {0.0, 0.0, 0.0}, rv[3] = {0.0, 0.0, 0.0};
             if (live && k < L) {
                 const double p0 = x[3 * k], p1 = x[3 * k + 1], p2 = x[3 * k + 2];
                 rv[0] = ref[3 * k];
@@ -211,25 +239,21 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ ce
             double et[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (MODE == kGramBF16x3) {
+                if constexpr (MODE == kGramBF16x3) {
+                    // lane-half h = w / 8, element w % 8 of planes 2c (hi), 2c+1 (lo)
                     const uint16_t hi = bf16_rne((float)ev[c]);
                     const double dhi = bf16_to_double(hi);
                     const uint16_t lo = bf16_rne((float)(ev[c] - dhi));
                     et[c] = dhi + bf16_to_double(lo);
-                    // planes 2c (hi) and 2c+1 (lo); element e of the lane's 8
-                    const int w = e >> 1, sh = (e & 1) * 16;
-                    if ((e & 1) == 0) {
-                        words[2 * c][w] = hi;
-                        words[2 * c + 1][w] = lo;
-                    } else {
-                        words[2 * c][w] |= (uint32_t)hi << sh;
-                        words[2 * c + 1][w] |= (uint32_t)lo << sh;
-                    }
+                    const int h = w >> 3, e = w & 7;
+                    img16[((kbl * 6 + 2 * c) * 2 + h) * 8 + e] = hi;
+                    img16[((kbl * 6 + 2 * c + 1) * 2 + h) * 8 + e] = lo;
                 } else {
+                    // lane-half h = w % 2, step s = w / 2 of plane c
                     const float f = (float)ev[c];
                     et[c] = (double)f;
-                    // plane c holds 8 floats per lane = two uint4: words[2c], words[2c+1]
-                    words[2 * c + (e >> 2)][e & 3] = __float_as_uint(f);
+                    const int h = w & 1, st = w >> 1;
+                    img32[((kbl * 3 + c) * 2 + h) * 8 + st] = f;
                 }
             }
             g += et[0] * et[0] + et[1] * et[1] + et[2] * et[2];
@@ -238,16 +262,19 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ ce
 #pragma unroll
                 for (int q = 0; q < 3; ++q) C[3 * p + q] += rv[p] * et[q];
         }
-        const int plane_lane = crow + 32 * h;
-        uint4* blk = panels + ((int64_t)cb * n_kblk + kb) * kPanelBlockU4;
-#pragma unroll
-        for (int t = 0; t < 6; ++t) {
-            uint4 v = make_uint4(words[t][0], words[t][1], words[t][2], words[t][3]);
-            if (MODE == kGramBF16x3)
-                blk[t * 64 + plane_lane] = v;                            // plane t
-            else
-                blk[(t >> 1) * 128 + plane_lane * 2 + (t & 1)] = v;      // plane t/2, half t&1
+        __syncthreads();
+        for (int u = lane; u < nkb * 12; u += 64) {
+            const int kbl = u / 12, r = u % 12;
+            uint4* blk = panels + ((int64_t)cb * n_kblk + kb0 + kbl) * kPanelBlockU4;
+            if constexpr (MODE == kGramBF16x3) {
+                const int p = r >> 1, h = r & 1;  // plane p, lane-half h
+                blk[p * 64 + crow + 32 * h] = img[wv][(kbl * 6 + p) * 2 + h];
+            } else {
+                const int c = r >> 2, h = (r >> 1) & 1, half = r & 1;  // 8 floats = 2 words
+                blk[c * 128 + (crow + 32 * h) * 2 + half] = img[wv][((kbl * 3 + c) * 2 + h) * 2 + half];
+            }
         }
+        __syncthreads();
     }
     g = wave_sum(g);
 #pragma unroll
@@ -327,7 +354,8 @@ void launch_centre(const double* xyz, int n, int L, double* centred, hipStream_t
 
 void launch_fit(const double* centred, int n, int L, const double* ref, double* rot, hipStream_t s)
 {
-    fit_kernel<<<(n + 3) / 4, 256, 0, s>>>(centred, n, L, ref, rot);
+    fit_cov_kernel<<<(n + 3) / 4, 256, 0, s>>>(centred, n, L, ref, rot);
+    fit_rot_kernel<<<(n + 63) / 64, 64, 0, s>>>(rot, n);
 }
 
 void launch_mean(const double* centred, const double* rot, int n, int L, double* partial,
@@ -335,7 +363,7 @@ void launch_mean(const double* centred, const double* rot, int n, int L, double*
 {
     const int slices = (n + kMeanSlice - 1) / kMeanSlice;
     mean_partial_kernel<<<slices, 256, 0, s>>>(centred, rot, n, L, partial);
-    mean_final_kernel<<<(3 * L + 255) / 256, 256, 0, s>>>(partial, slices, n, L, ref_out);
+    mean_final_kernel<<<(3 * L + 3) / 4, 256, 0, s>>>(partial, slices, n, L, ref_out);
     ref_gram_kernel<<<1, 256, 0, s>>>(ref_out, L);
 }
 

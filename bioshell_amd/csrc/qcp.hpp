@@ -36,6 +36,23 @@ namespace bs {
 
 #define BS_HD __host__ __device__ __forceinline__
 
+// a / b for the Newton steps: on the device v_rcp_f64 refined by two
+// Newton-Raphson steps (relative error ~1e-16 after the second), instead of the
+// IEEE division sequence; a Newton step tolerates any error far below its size.
+BS_HD double newton_div(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    return a * r;
+#else
+    return a / b;
+#endif
+}
+
 template <class T>
 BS_HD T det3(const T* m)
 {
@@ -73,13 +90,14 @@ BS_HD T det_key(const T* M)
 // symmetric matrix are perfectly conditioned, so this stays accurate where the
 // characteristic polynomial has a (near-)double root.
 // Returns the unit eigenvector of the largest eigenvalue of symmetric A.
-BS_HD void top_eigvec4(double A[4][4], double q[4], double* lambda)
+template <class T>
+BS_HD void top_eigvec4(T A[4][4], T q[4], T* lambda)
 {
-    double V[4][4];
+    T V[4][4];
     for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) V[r][c] = (r == c) ? 1.0 : 0.0;
+        for (int c = 0; c < 4; ++c) V[r][c] = (r == c) ? T(1) : T(0);
     for (int sweep = 0; sweep < 30; ++sweep) {
-        double off = 0.0, diag = 0.0;
+        T off = T(0), diag = T(0);
         for (int r = 0; r < 4; ++r)
             for (int c = 0; c < 4; ++c) {
                 if (r == c)
@@ -87,26 +105,28 @@ BS_HD void top_eigvec4(double A[4][4], double q[4], double* lambda)
                 else
                     off += A[r][c] * A[r][c];
             }
-        if (off <= 1e-30 * diag || off == 0.0) break;
+        // off-diagonal mass below 1e-24 (FP64) of the diagonal: eigenvalues are
+        // then exact to ~1e-24 relative and eigenvectors to ~1e-12
+        if (off <= (sizeof(T) == 8 ? T(1e-24) : T(1e-12)) * diag || off == T(0)) break;
         for (int p = 0; p < 3; ++p)
             for (int r = p + 1; r < 4; ++r) {
-                const double apr = A[p][r];
-                if (apr == 0.0) continue;
-                const double th = (A[r][r] - A[p][p]) / (2.0 * apr);
-                const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                const T apr = A[p][r];
+                if (apr == T(0)) continue;
+                const T th = (A[r][r] - A[p][p]) / (T(2) * apr);
+                const T t = (th >= T(0) ? T(1) : T(-1)) / (fabs(th) + sqrt(th * th + T(1)));
+                const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
                 for (int k = 0; k < 4; ++k) {
-                    const double akp = A[k][p], akr = A[k][r];
+                    const T akp = A[k][p], akr = A[k][r];
                     A[k][p] = c * akp - s * akr;
                     A[k][r] = s * akp + c * akr;
                 }
                 for (int k = 0; k < 4; ++k) {
-                    const double apk = A[p][k], ark = A[r][k];
+                    const T apk = A[p][k], ark = A[r][k];
                     A[p][k] = c * apk - s * ark;
                     A[r][k] = s * apk + c * ark;
                 }
                 for (int k = 0; k < 4; ++k) {
-                    const double vkp = V[k][p], vkr = V[k][r];
+                    const T vkp = V[k][p], vkr = V[k][r];
                     V[k][p] = c * vkp - s * vkr;
                     V[k][r] = s * vkp + c * vkr;
                 }
@@ -115,7 +135,7 @@ BS_HD void top_eigvec4(double A[4][4], double q[4], double* lambda)
     int best = 0;
     for (int r = 1; r < 4; ++r)
         if (A[r][r] > A[best][best]) best = r;
-    double nq = 0.0;
+    T nq = T(0);
     for (int r = 0; r < 4; ++r) nq += V[r][best] * V[r][best];
     nq = sqrt(nq);
     for (int r = 0; r < 4; ++r) q[r] = V[r][best] / nq;
@@ -123,9 +143,10 @@ BS_HD void top_eigvec4(double A[4][4], double q[4], double* lambda)
 }
 
 // Horn's key matrix of M as a 4x4 array.
-BS_HD void key_matrix(const double* M, double K[4][4])
+template <class T>
+BS_HD void key_matrix(const T* M, T K[4][4])
 {
-    const double xx = M[0], xy = M[1], xz = M[2], yx = M[3], yy = M[4], yz = M[5], zx = M[6],
+    const T xx = M[0], xy = M[1], xz = M[2], yx = M[3], yy = M[4], yz = M[5], zx = M[6],
                  zy = M[7], zz = M[8];
     K[0][0] = xx + yy + zz; K[0][1] = yz - zy;       K[0][2] = zx - xz;       K[0][3] = xy - yx;
     K[1][0] = yz - zy;      K[1][1] = xx - yy - zz;  K[1][2] = xy + yx;       K[1][3] = zx + xz;
@@ -145,13 +166,13 @@ BS_HD T qcp_rmsd(const T* M, T Ga, T Gb, int n, int max_iter = 50)
     const T c1 = T(-8) * det3(M);
     const T c0 = det_key(M);
     const T E0 = T(0.5) * (Ga + Gb);
-    T l = E0;
+    T l = E0, dp = T(0);
     const T tol = sizeof(T) == 8 ? T(1e-11) : T(1e-6);
     int it = 0;
     for (; it < max_iter; ++it) {
         const T l2 = l * l;
         const T p = ((l2 + c2) * l + c1) * l + c0;
-        const T dp = (T(4) * l2 + T(2) * c2) * l + c1;
+        dp = (T(4) * l2 + T(2) * c2) * l + c1;
         if (dp == T(0)) break;
         const T step = p / dp;
         l -= step;
@@ -160,15 +181,20 @@ BS_HD T qcp_rmsd(const T* M, T Ga, T Gb, int n, int max_iter = 50)
             break;
         }
     }
-    // Near-degenerate polish (FP64 only): linear convergence means a (near-)
-    // double root, which the polynomial pins to ~sqrt(eps) relative only.
-    if constexpr (sizeof(T) == 8) {
-      if (it > 8) {
-        double K[4][4], q[4], lam;
-        key_matrix(reinterpret_cast<const double*>(M), K);
+    // Near-degenerate polish: linear convergence means a (near-) double root,
+    // which the polynomial pins to ~sqrt(eps) relative only.
+    // In FP32 the polynomial is too noisy near a double root for the step test
+    // to be trusted, and sqrt() amplifies any error in a small E0 - lambda.
+    // P'(lambda) is the product of the gaps to the other eigenvalues: small
+    // relative to its terms means a near-double root.
+    const T scale = fabs(T(4) * l * l * l) + fabs(T(2) * c2 * l) + fabs(c1);
+    bool polish = it > 8 || fabs(dp) <= T(1e-4) * scale;
+    if constexpr (sizeof(T) == 4) polish = polish || (E0 - l) < T(1e-3) * E0;
+    if (polish) {
+        T K[4][4], q[4], lam;
+        key_matrix(M, K);
         top_eigvec4(K, q, &lam);
-        l = T(lam);
-      }
+        l = lam;
     }
     T r2 = T(2) * (E0 - l) / T(n);
     return sqrt(r2 > T(0) ? r2 : T(0));
@@ -188,11 +214,14 @@ struct ShiftedOut {
     bool converged;
 };
 
-// Solves for the largest root of det(mu I - K') given the pair's residual
-// Gram E (row-major, E[3p+q] = sum_k e_i,kp e_j,kq) and both conformations'
-// constants.
-BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj,
-                             int max_iter = 40)
+// Coefficients of det(mu I - K') = mu^4 + p3 mu^3 + p2 mu^2 + p1 mu + p0 for
+// one pair, from its residual Gram E (row-major, E[3p+q] = sum_k e_i,kp e_j,kq)
+// and both conformations' constants.
+struct ShiftedPoly {
+    double p3, p2, p1, p0, H, aa;
+};
+
+BS_HD ShiftedPoly shifted_poly(const double* E, const double* ci, const double* cj)
 {
     const double trE = E[0] + E[4] + E[8];
     // B = Bh_i + Bh_j + (E + E^T - 2 tr E I)
@@ -206,8 +235,6 @@ BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj
     const double a0 = (cj[6] - ci[6]) + (E[5] - E[7]);
     const double a1 = (cj[7] - ci[7]) + (E[6] - E[2]);
     const double a2 = (cj[8] - ci[8]) + (E[1] - E[3]);
-    const double H = (ci[9] + cj[9]) - trE;
-
     // invariants of B and its adjugate
     const double A00 = b11 * b22 - b12 * b12;
     const double A11 = b00 * b22 - b02 * b02;
@@ -223,41 +250,119 @@ BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj
                        2.0 * (b01 * a0 * a1 + b02 * a0 * a2 + b12 * a1 * a2);
     const double aAa = A00 * a0 * a0 + A11 * a1 * a1 + A22 * a2 * a2 +
                        2.0 * (A01 * a0 * a1 + A02 * a0 * a2 + A12 * a1 * a2);
-    // det(mu I - K') = mu^4 + p3 mu^3 + p2 mu^2 + p1 mu + p0
-    const double p3 = -t1;
-    const double p2 = t2 - aa;
-    const double p1 = t1 * aa - aBa - t3;
-    const double p0 = -aAa;
+    ShiftedPoly s;
+    s.p3 = -t1;
+    s.p2 = t2 - aa;
+    s.p1 = t1 * aa - aBa - t3;
+    s.p0 = -aAa;
+    s.H = (ci[9] + cj[9]) - trE;
+    s.aa = aa;
+    return s;
+}
 
-    double mu = H > 0.0 ? H : 0.0;
-    const double floor_tol = 1e-15 * (H > 0.0 ? H : 0.0);
-    int it = 0;
-    bool conv = false;
-    for (; it < max_iter; ++it) {
-        const double p = (((mu + p3) * mu + p2) * mu + p1) * mu + p0;
-        const double dp = ((4.0 * mu + 3.0 * p3) * mu + 2.0 * p2) * mu + p1;
-        if (!(dp > 0.0)) {  // at or below a stationary point: stop, flag
-            break;
-        }
-        const double step = p / dp;
-        mu -= step;
-        if (fabs(step) <= 1e-13 * fabs(mu) + floor_tol) {
-            conv = true;
-            ++it;
-            break;
+BS_HD double poly_at(const ShiftedPoly& s, double m)
+{
+    return (((m + s.p3) * m + s.p2) * m + s.p1) * m + s.p0;
+}
+BS_HD double dpoly_at(const ShiftedPoly& s, double m)
+{
+    return ((4.0 * m + 3.0 * s.p3) * m + 2.0 * s.p2) * m + s.p1;
+}
+
+// Start: near mu = 0 the quartic is close to p1 mu + p0, so -p0/p1 is the
+// root to first order.  It is used only when it provably bounds every real
+// root from above: all Taylor coefficients of the polynomial at that point
+// (P, P', P''/2, P'''/6, 1) are >= 0.  Newton from an upper bound of a
+// real-rooted polynomial converges monotonically to the largest root;
+// otherwise the start is the upper bound H (lambda <= E0).
+BS_HD double shifted_start(const ShiftedPoly& s)
+{
+    const double Hp = s.H > 0.0 ? s.H : 0.0;
+    double mu = Hp;
+    if (s.p1 > 0.0) {
+        const double m = -newton_div(s.p0, s.p1);
+        if (m >= 0.0 && m < Hp) {
+            const double t0 = poly_at(s, m);
+            const double t1d = dpoly_at(s, m);
+            const double t2d = (6.0 * m + 3.0 * s.p3) * m + s.p2;
+            const double t3d = 4.0 * m + s.p3;
+            if (t0 >= 0.0 && t1d >= 0.0 && t2d >= 0.0 && t3d >= 0.0) mu = m;
         }
     }
-    if (aa == 0.0 && !(mu > 0.0)) {  // exactly aligned pair: mu = max(0, lmax(B))
-        conv = true;
+    return mu;
+}
+
+// Newton state of one pair.
+struct ShiftedIter {
+    double mu, dp;
+    int it;
+    bool conv, stuck;
+};
+
+// One Newton step; returns true when converged (or stuck at a stationary point).
+BS_HD bool shifted_step(const ShiftedPoly& s, ShiftedIter& st)
+{
+    const double p = poly_at(s, st.mu);
+    st.dp = dpoly_at(s, st.mu);
+    ++st.it;
+    if (!(st.dp > 0.0)) {  // at or below a stationary point: stop, flag
+        st.stuck = true;
+        return true;
     }
-    if (it > 8) conv = false;  // linear convergence: (near-)double root, re-evaluate in FP64
+    const double step = newton_div(p, st.dp);
+    st.mu -= step;
+    const double Hp = s.H > 0.0 ? s.H : 0.0;
+    st.conv = fabs(step) <= 1e-13 * fabs(st.mu) + 1e-15 * Hp;
+    return st.conv;
+}
+
+BS_HD ShiftedOut shifted_finish(const ShiftedPoly& s, const ShiftedIter& st)
+{
+    bool conv = st.conv && !st.stuck;
+    if (s.aa == 0.0 && !(st.mu > 0.0)) conv = true;  // exactly aligned pair: mu = max(0, lmax(B))
+    // near-double root (a small P' at the root): re-evaluate in FP64
+    const double m = st.mu;
+    const double scale =
+        fabs(4.0 * m * m * m) + fabs(3.0 * s.p3 * m * m) + fabs(2.0 * s.p2 * m) + fabs(s.p1);
+    if (fabs(st.dp) <= 1e-4 * scale) conv = false;
     ShiftedOut o;
-    const double d = H - mu;
+    const double d = s.H - m;
     o.rmsd2_L = 2.0 * (d > 0.0 ? d : 0.0);
-    o.H = H;
-    o.iterations = it;
+    o.H = s.H;
+    o.iterations = st.it;
     o.converged = conv;
     return o;
+}
+
+// Solves for the largest root of det(mu I - K') for one pair.
+BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj,
+                             int max_iter = 40)
+{
+    const ShiftedPoly s = shifted_poly(E, ci, cj);
+    ShiftedIter st{shifted_start(s), 0.0, 0, false, false};
+    while (st.it < max_iter && !shifted_step(s, st)) {
+    }
+    return shifted_finish(s, st);
+}
+
+// Two pairs sharing a column, solved in lockstep: coefficient evaluation and
+// the first two Newton steps are straight-line code for both, so the two
+// dependency chains interleave; only unconverged pairs continue alone.
+BS_HD void qcp_shifted2(const double* E0, const double* ci0, const double* E1, const double* ci1,
+                        const double* cj, ShiftedOut& o0, ShiftedOut& o1, int max_iter = 40)
+{
+    const ShiftedPoly s0 = shifted_poly(E0, ci0, cj);
+    const ShiftedPoly s1 = shifted_poly(E1, ci1, cj);
+    ShiftedIter t0{shifted_start(s0), 0.0, 0, false, false};
+    ShiftedIter t1{shifted_start(s1), 0.0, 0, false, false};
+    bool d0 = shifted_step(s0, t0);
+    bool d1 = shifted_step(s1, t1);
+    if (!d0) d0 = shifted_step(s0, t0);
+    if (!d1) d1 = shifted_step(s1, t1);
+    while (!d0 && t0.it < max_iter) d0 = shifted_step(s0, t0);
+    while (!d1 && t1.it < max_iter) d1 = shifted_step(s1, t1);
+    o0 = shifted_finish(s0, t0);
+    o1 = shifted_finish(s1, t1);
 }
 
 }  // namespace bs

@@ -123,8 +123,9 @@ double or_lambda_jacobi4(const double M[9]);
  * stops when the step is below 1e-11 relative (or after 50 steps).
  * Near-degenerate polish: Newton converges only linearly at a double root
  * (e.g. collinear sets), where the polynomial fixes the root to ~sqrt(eps)
- * only; when more than 8 steps were needed the eigenvalue is taken from a
- * Jacobi solve of K instead, which is accurate to eps |K| regardless. */
+ * only; when more than 8 steps were needed, or P'(lambda) is below 1e-4 of
+ * the size of its terms, the eigenvalue is taken from a Jacobi solve of K
+ * instead, which is accurate to eps |K| regardless. */
 double or_qcp_lambda(const double M[9], double E0, int* iterations)
 {
     double K[16];
@@ -135,19 +136,22 @@ double or_qcp_lambda(const double M[9], double E0, int* iterations)
     const double c1 = -8.0 * det3(M);
     const double c0 = det4(K);
 
-    double l = E0;
+    double l = E0, dp = 0.0;
     int it = 0;
     for (; it < 50; ++it) {
         const double l2 = l * l;
         const double p = ((l2 + c2) * l + c1) * l + c0;
-        const double dp = (4.0 * l2 + 2.0 * c2) * l + c1;
+        dp = (4.0 * l2 + 2.0 * c2) * l + c1;
         if (dp == 0.0) break;
         const double step = p / dp;
         l -= step;
         if (fabs(step) <= 1e-11 * fabs(l)) { ++it; break; }
     }
     if (iterations) *iterations = it;
-    if (it > 8) return or_lambda_jacobi4(M);
+    /* P'(lambda) is the product of the gaps to the other three eigenvalues: a
+     * small P' relative to its terms means a near-double root */
+    const double scale = fabs(4.0 * l * l * l) + fabs(2.0 * c2 * l) + fabs(c1);
+    if (it > 8 || fabs(dp) <= 1e-4 * scale) return or_lambda_jacobi4(M);
     return l;
 }
 

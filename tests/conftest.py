@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+try:
+    # PyTorch-ROCm ships its own HIP runtime (libamdhip64.so.7).  Loading it
+    # before libbioshell_gpu.so makes the library bind to that same runtime, so
+    # one process never holds two HIP runtimes.
+    import torch  # noqa: F401
+except Exception:  # torch is only needed by the device-pointer tests
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
