@@ -40,10 +40,11 @@ def cpu_baseline(x: np.ndarray, target_s: float, threads: int) -> dict:
     from oracle import oracle as O
 
     n = x.shape[0]
+    probe = min(n, 64)
     t0 = time.perf_counter()
-    O.all_vs_all_rows(x, 0, 8, nthreads=threads)
+    O.all_vs_all_rows(x, 0, probe, nthreads=threads)
     dt = time.perf_counter() - t0
-    probe_pairs = sum(n - 1 - i for i in range(8))
+    probe_pairs = sum(n - 1 - i for i in range(probe))
     rate = probe_pairs / max(dt, 1e-6)
     want_pairs = rate * target_s
     R, pairs = 0, 0
@@ -99,17 +100,17 @@ def main() -> None:
     for _ in range(args.warmup):
         step(B.Method.BF16X3_GRAM, d_out)
     torch.cuda.synchronize()
-    main_ms, prep_ms, fb_ms, flagged = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(B.Method.BF16X3_GRAM, d_out)
-        st = ctx.last_stats()
-        main_ms.append(st["main_ms"])
-        prep_ms.append(st["prep_ms"])
-        fb_ms.append(st["fallback_ms"])
-        flagged.append(st["flagged_pairs"])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # per-step device times from the library's HIP events (on its stream)
+    hist = ctx.history(args.steps)
+    main_ms = [h["main_ms"] for h in hist]
+    prep_ms = [h["prep_ms"] for h in hist]
+    fb_ms = [h["fallback_ms"] for h in hist]
+    flagged = [h["flagged_pairs"] for h in hist]
     ms_per_step = dt * 1e3 / args.steps
     value = pairs / (dt / args.steps)
     log(f"bf16x3 gram: {ms_per_step:.3f} ms/step, kernel {np.mean(main_ms):.3f} ms")
@@ -119,12 +120,11 @@ def main() -> None:
         step(B.Method.FP32_PLAIN, d_plain)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain_kernel = []
     for _ in range(args.plain_steps):
         step(B.Method.FP32_PLAIN, d_plain)
-        plain_kernel.append(ctx.last_stats()["main_ms"])
     torch.cuda.synchronize()
     plain_dt = (time.perf_counter() - t0) / args.plain_steps
+    plain_kernel = [h["main_ms"] for h in ctx.history(args.plain_steps)]
     plain_value = pairs / plain_dt
     log(f"fp32 plain: {plain_dt * 1e3:.3f} ms/step")
 

@@ -49,6 +49,7 @@ SIGNATURES = {
     "bs_context_stream": (c_vp, [c_vp]),
     "bs_context_last_stats": (c_status, [c_vp, ctypes.POINTER(RunStats)]),
     "bs_context_bytes_in_use": (c_size, [c_vp]),
+    "bs_context_history": (c_status, [c_vp, ctypes.POINTER(RunStats), c_size, ctypes.POINTER(c_size)]),
     "bs_crmsd_pair_f64": (c_status, [c_vp, c_dp, c_dp, c_size, c_dp]),
     "bs_superpose_f64": (c_status, [c_vp, c_dp, c_dp, c_size, c_dp, c_dp, c_dp]),
     "bs_crmsd_all_vs_all": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_vp, ctypes.c_int]),

@@ -117,6 +117,13 @@ class Context:
         check(_native.lib().bs_context_last_stats(self.handle, ctypes.byref(st)))
         return st.as_dict()
 
+    def history(self, max_calls: int = 64) -> list[dict]:
+        """Stats of the most recent calls, oldest first (synchronises on them)."""
+        buf = (_native.RunStats * max_calls)()
+        n = ctypes.c_size_t()
+        check(_native.lib().bs_context_history(self.handle, buf, max_calls, ctypes.byref(n)))
+        return [buf[k].as_dict() for k in range(n.value)]
+
     def bytes_in_use(self) -> int:
         return _native.lib().bs_context_bytes_in_use(self.handle)
 
@@ -239,7 +246,11 @@ def crmsd_all_vs_all(models, method: Method = Method.BF16X3_GRAM, out_f64: bool 
 def crmsd_all_vs_all_device(d_xyz: int, n_conf: int, n_atoms: int, d_out: int,
                             method: Method = Method.BF16X3_GRAM, out_f64: bool = False,
                             ctx: Context | None = None) -> None:
-    """All-vs-all on device pointers (e.g. ``tensor.data_ptr()``)."""
+    """All-vs-all on device pointers (e.g. ``tensor.data_ptr()``).
+
+    Asynchronous on the context's stream: call ``ctx.synchronize()`` (or a
+    device-wide synchronisation) before reading the output.
+    """
     ctx = ctx or default_context()
     check(_native.lib().bs_crmsd_all_vs_all_device(ctx.handle, ctypes.c_void_p(d_xyz), n_conf, n_atoms,
                                                    int(method), ctypes.c_void_p(d_out), int(out_f64)))

@@ -1,5 +1,6 @@
 // api.cpp -- the extern "C" entry points declared in include/bioshell_gpu.h.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -70,6 +71,9 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         auto ctx = std::make_unique<bs_context>();
         ctx->device = device;
         ctx->n_cu = prop.multiProcessorCount;
+        ctx->gsc.max_wg = 2 * ctx->n_cu;
+        if (const char* e = std::getenv("BIOSHELL_GRAM_SKEW")) ctx->gsc.skew_sleeps = std::atoi(e);
+        if (const char* e = std::getenv("BIOSHELL_GRAM_VARIANT")) ctx->gsc.variant = std::atoi(e);
         ctx->budget.set_cap(hbm_budget_bytes);
         BS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->prep.bind(&ctx->budget);
@@ -77,10 +81,6 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         ctx->ws_xyz.bind(&ctx->budget);
         ctx->ws_out.bind(&ctx->budget);
         ctx->ws_misc.bind(&ctx->budget);
-        ctx->t_total.create();
-        ctx->t_prep.create();
-        ctx->t_main.create();
-        ctx->t_fallback.create();
         *out = ctx.release();
     });
 }
@@ -92,12 +92,10 @@ void bs_context_destroy(bs_context* ctx)
         DeviceGuard dg(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         ctx->prep.reset();
-        ctx->t_total.destroy();
-        ctx->t_prep.destroy();
-        ctx->t_main.destroy();
-        ctx->t_fallback.destroy();
-        if (ctx->gsc.h_count) (void)hipHostFree(ctx->gsc.h_count);
-        ctx->gsc.h_count = nullptr;
+        for (auto& r : ctx->ring) r.destroy();
+        ctx->ring.clear();
+        if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+        ctx->h_counts = nullptr;
         ctx->gsc.flags.reset();
         ctx->gsc.tiles.reset();
         ctx->ws_xyz.reset();
@@ -123,7 +121,26 @@ bs_status bs_context_last_stats(const bs_context* ctx, bs_run_stats* out)
     return guard([&] {
         check_ctx(ctx);
         BS_REQUIRE(out != nullptr, "out is null");
-        *out = ctx->last;
+        *out = bs_run_stats{};
+        if (ctx->ring_used == 0) return;
+        const size_t k = (ctx->ring_next + ctx->ring.size() - 1) % ctx->ring.size();
+        *out = ctx->ring[k].read();
+    });
+}
+
+bs_status bs_context_history(const bs_context* ctx, bs_run_stats* out, size_t max_calls,
+                             size_t* n_out)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        BS_REQUIRE(n_out != nullptr && (out != nullptr || max_calls == 0), "null pointer argument");
+        const size_t n = std::min(max_calls, ctx->ring_used);
+        const size_t R = ctx->ring.size();
+        for (size_t q = 0; q < n; ++q) {
+            const size_t k = (ctx->ring_next + R - n + q) % R;
+            out[q] = ctx->ring[k].read();
+        }
+        *n_out = n;
     });
 }
 
@@ -196,10 +213,7 @@ bs_status bs_crmsd_all_vs_all(bs_context* ctx, const double* xyz, size_t n_conf,
         check_ctx(ctx);
         check_shape(n_conf, n_atoms);
         BS_REQUIRE(valid_method(method), "unknown method %d", (int)method);
-        if (n_conf < 2) {
-            ctx->last = bs_run_stats{};
-            return;
-        }
+        if (n_conf < 2) return;
         BS_REQUIRE(xyz && out_upper, "null pointer argument");
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);

@@ -53,8 +53,37 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
     BS_HIP(hipGetLastError());
 }
 
-uint64_t run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
-                  hipStream_t s, Timer* t_main, Timer* t_fb, int* launches)
+void RunRecord::create()
+{
+    for (auto& e : ev) BS_HIP(hipEventCreate(&e));
+}
+
+void RunRecord::destroy()
+{
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+bs_run_stats RunRecord::read() const
+{
+    bs_run_stats st = base;
+    if (!used) return st;
+    BS_HIP(hipEventSynchronize(ev[2 * kRecTotal + 1]));
+    auto span = [&](int k) {
+        float v = 0.0f;
+        BS_HIP(hipEventElapsedTime(&v, ev[2 * k], ev[2 * k + 1]));
+        return (double)v;
+    };
+    st.total_ms = span(kRecTotal);
+    st.prep_ms = span(kRecPrep);
+    st.main_ms = span(kRecMain);
+    st.fallback_ms = span(kRecFallback);
+    st.flagged_pairs = gram && h_count ? *h_count : 0;
+    return st;
+}
+
+void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
+              hipStream_t s, RunRecord* rec)
 {
     BS_REQUIRE(A.L == B.L && A.mode == B.mode, "prepared sets disagree on atom count or mode");
     const int tiles_r = A.n_pad / kTileRows, tiles_c = B.n_pad / kTileRows;
@@ -70,125 +99,135 @@ uint64_t run_gram(const Prepared& A, const Prepared& B, const GramRun& run, Gram
         sc.tiles_key = key;
         sc.tiles_n = nt;
     }
+    // The list holds up to 4M pairs; flagged pairs beyond that are re-evaluated
+    // in FP64 inside the Gram kernel, so nothing is dropped and no host check
+    // is needed.
     const uint64_t pairs = run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * B.n;
-    sc.max_flagged = (int)std::max<uint64_t>(
-        (uint64_t)sc.max_flagged, std::max<uint64_t>(4096, std::min<uint64_t>(pairs, 1u << 22)));
-    if (!sc.h_count) BS_HIP(hipHostMalloc((void**)&sc.h_count, 64, hipHostMallocDefault));
+    const int want = (int)std::max<uint64_t>(4096, std::min<uint64_t>(pairs, 1u << 22));
+    sc.max_flagged = std::max(sc.max_flagged, want);
+    uint32_t* flags = static_cast<uint32_t*>(
+        sc.flags.ensure((size_t)(2 * (size_t)sc.max_flagged + 64) * 4, "flagged pairs"));
+    uint32_t* count = flags + 2 * (size_t)sc.max_flagged;
+    launch_zero_u32(count, 1, s);
 
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        uint32_t* flags = static_cast<uint32_t*>(
-            sc.flags.ensure((size_t)(2 * (size_t)sc.max_flagged + 64) * 4, "flagged pairs"));
-        uint32_t* count = flags + 2 * (size_t)sc.max_flagged;
-        launch_zero_u32(count, 1, s);
-
-        GramArgs g{};
-        g.panels_a = A.panels.get();
-        g.panels_b = B.panels.get();
-        g.consts_a = A.consts.as<double>();
-        g.consts_b = B.consts.as<double>();
-        g.lda = A.n_pad;
-        g.ldb = B.n_pad;
-        g.n_rows = A.n;
-        g.n_cols = B.n;
-        g.L = A.L;
-        g.n_kblk = A.n_kblk;
-        g.tiles = sc.tiles.as<uint32_t>();
-        g.n_tiles = sc.tiles_n;
-        g.triangle = run.triangle ? 1 : 0;
-        g.out_ld = run.out_ld;
-        g.out = run.d_out;
-        g.out_f64 = run.out_f64 ? 1 : 0;
-        g.flag_pairs = flags;
-        g.flag_count = count;
-        g.max_flagged = sc.max_flagged;
-        // Error model of the residual Gram: bf16 hi/lo represents e to 2^-16
-        // relative (the dropped lo*lo term is bounded by 2^-16 sqrt(g_i g_j));
-        // FP32 accumulation adds ~2^-24 sqrt(#accumulations) sqrt(g_i g_j).
-        const double n_acc = A.mode == kGramBF16x3 ? 3.0 * A.n_kblk : 8.0 * A.n_kblk;
-        g.kappa_dup = 8.0;
-        g.u_rep = A.mode == kGramBF16x3 ? std::ldexp(1.0, -16) : std::ldexp(1.0, -24);
-        // Error of H - mu in units of sqrt(g_i g_j): FP32 accumulation
-        // (~2^-24 per accumulation, random-walk) plus, for the split, the
-        // dropped lo*lo products (|sum lo_i lo_j| <= 2^-16 sqrt(g_i g_j), a sum
-        // of 3L terms of random sign: 3 sigma = 3/sqrt(3L) of that bound).
-        g.acc_coef = 2.0 * std::ldexp(1.0, -24) * std::sqrt(n_acc);
-        if (A.mode == kGramBF16x3)
-            g.acc_coef += g.u_rep * std::min(1.0, 3.0 / std::sqrt(3.0 * A.L));
-        g.tau = 2e-5;
-        // Rounding e to the split representation moves each coordinate by up
-        // to u_rep |e|; its effect on RMSD has a standard deviation of about
-        // u_rep sqrt(2 (g_i + g_j)) / (3 L).  Pairs whose 6-sigma value could
-        // exceed 5e-5 A (tiny L, large residuals) are re-evaluated in FP64.
-        {
-            const double t = 5e-5 * 3.0 * A.L / (6.0 * std::sqrt(2.0) * g.u_rep);
-            g.rep_g_max = t * t;
-        }
-
-        if (t_main) t_main->start(s);
-        launch_gram(g, A.mode, s);
-        if (t_main) t_main->stop(s);
-        BS_HIP(hipGetLastError());
-        if (launches) ++*launches;
-        if (t_fb) t_fb->start(s);
-        launch_fallback(A.centred.as<double>(), B.centred.as<double>(), A.L, flags, count,
-                        sc.max_flagged, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, s);
-        if (t_fb) t_fb->stop(s);
-        BS_HIP(hipGetLastError());
-        BS_HIP(hipMemcpyAsync(sc.h_count, count, 4, hipMemcpyDeviceToHost, s));
-        BS_HIP(hipStreamSynchronize(s));
-        const uint32_t c = *sc.h_count;
-        if ((int)c <= sc.max_flagged) return c;
-        // capacity exceeded: grow and run again (every output is rewritten)
-        sc.max_flagged = (int)std::min<uint64_t>((uint64_t)c + c / 4, 0x7FFFFFFF / 2);
+    GramArgs g{};
+    g.panels_a = A.panels.get();
+    g.panels_b = B.panels.get();
+    g.consts_a = A.consts.as<double>();
+    g.consts_b = B.consts.as<double>();
+    g.lda = A.n_pad;
+    g.ldb = B.n_pad;
+    g.n_rows = A.n;
+    g.n_cols = B.n;
+    g.L = A.L;
+    g.n_kblk = A.n_kblk;
+    g.tiles = sc.tiles.as<uint32_t>();
+    g.n_tiles = sc.tiles_n;
+    g.triangle = run.triangle ? 1 : 0;
+    g.out_ld = run.out_ld;
+    g.out = run.d_out;
+    g.out_f64 = run.out_f64 ? 1 : 0;
+    g.centred_a = A.centred.as<double>();
+    g.centred_b = B.centred.as<double>();
+    g.flag_pairs = flags;
+    g.flag_count = count;
+    g.max_flagged = sc.max_flagged;
+    // Error model of the residual Gram: bf16 hi/lo represents e to 2^-16
+    // relative (the dropped lo*lo term is bounded by 2^-16 sqrt(g_i g_j));
+    // FP32 accumulation adds ~2^-24 sqrt(#accumulations) sqrt(g_i g_j).
+    const double n_acc = A.mode == kGramBF16x3 ? 3.0 * A.n_kblk : 8.0 * A.n_kblk;
+    g.kappa_dup = 8.0;
+    g.u_rep = A.mode == kGramBF16x3 ? std::ldexp(1.0, -16) : std::ldexp(1.0, -24);
+    // Error of H - mu in units of sqrt(g_i g_j): FP32 accumulation
+    // (~2^-24 per accumulation, random-walk) plus, for the split, the
+    // dropped lo*lo products (|sum lo_i lo_j| <= 2^-16 sqrt(g_i g_j), a sum
+    // of 3L terms of random sign: 3 sigma = 3/sqrt(3L) of that bound).
+    g.acc_coef = 2.0 * std::ldexp(1.0, -24) * std::sqrt(n_acc);
+    if (A.mode == kGramBF16x3) g.acc_coef += g.u_rep * std::min(1.0, 3.0 / std::sqrt(3.0 * A.L));
+    g.tau = 2e-5;
+    // Rounding e to the split representation moves each coordinate by up
+    // to u_rep |e|; its effect on RMSD has a standard deviation of about
+    // u_rep sqrt(2 (g_i + g_j)) / (3 L).  Pairs whose 6-sigma value could
+    // exceed 5e-5 A (tiny L, large residuals) are re-evaluated in FP64.
+    {
+        const double t = 5e-5 * 3.0 * A.L / (6.0 * std::sqrt(2.0) * g.u_rep);
+        g.rep_g_max = t * t;
     }
-    fail(BS_ERR_INTERNAL, "flagged-pair list overflowed twice");
+    g.max_wg = sc.max_wg > 0 ? sc.max_wg : 512;
+    g.skew_sleeps = sc.skew_sleeps;
+
+    if (rec) rec->start(kRecMain, s);
+    if (sc.variant == 1)
+        launch_gram_pipe(g, A.mode, s);
+    else
+        launch_gram(g, A.mode, s);
+    if (rec) rec->stop(kRecMain, s);
+    BS_HIP(hipGetLastError());
+    if (rec) rec->start(kRecFallback, s);
+    launch_fallback(A.centred.as<double>(), B.centred.as<double>(), A.L, flags, count,
+                    sc.max_flagged, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, s);
+    if (rec) rec->stop(kRecFallback, s);
+    BS_HIP(hipGetLastError());
+    if (rec && rec->h_count)
+        BS_HIP(hipMemcpyAsync(rec->h_count, count, 4, hipMemcpyDeviceToHost, s));
+    if (rec) rec->gram = true;
 }
 
 void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m, void* d_out,
                  bool out_f64)
 {
     hipStream_t s = ctx->stream;
-    bs_run_stats st{};
-    st.pairs = n >= 2 ? (uint64_t)n * (n - 1) / 2 : 0;
-    if (n < 2) {
-        ctx->last = st;
-        return;
-    }
-    ctx->t_total.start(s);
-    if (!is_gram(m)) {
-        ctx->t_prep.start(s);
-        ctx->t_prep.stop(s);
-        ctx->t_main.start(s);
+    RunRecord* rec = ctx->begin_record();
+    rec->base.pairs = n >= 2 ? (uint64_t)n * (n - 1) / 2 : 0;
+    rec->start(kRecTotal, s);
+    if (n >= 2 && !is_gram(m)) {
+        rec->start(kRecPrep, s);
+        rec->stop(kRecPrep, s);
+        rec->start(kRecMain, s);
         launch_plain_avsa(d_xyz, n, L, m == BS_FP64_PLAIN, d_out, out_f64, s);
-        ctx->t_main.stop(s);
+        rec->stop(kRecMain, s);
         BS_HIP(hipGetLastError());
-        ctx->t_fallback.start(s);
-        ctx->t_fallback.stop(s);
-        ctx->t_total.stop(s);
-        BS_HIP(hipStreamSynchronize(s));
-        st.main_launches = 1;
-    } else {
-        ctx->t_prep.start(s);
+        rec->start(kRecFallback, s);
+        rec->stop(kRecFallback, s);
+        rec->base.main_launches = 1;
+    } else if (n >= 2) {
+        rec->start(kRecPrep, s);
         prepare(ctx->prep, d_xyz, n, L, mode_of(m), nullptr, s);
-        ctx->t_prep.stop(s);
+        rec->stop(kRecPrep, s);
         GramRun run;
         run.triangle = true;
         run.d_out = d_out;
         run.out_f64 = out_f64;
-        int launches = 0;
-        st.flagged_pairs = run_gram(ctx->prep, ctx->prep, run, ctx->gsc, s, &ctx->t_main,
-                                    &ctx->t_fallback, &launches);
-        ctx->t_total.stop(s);
-        BS_HIP(hipStreamSynchronize(s));
-        st.main_launches = (uint64_t)launches;
-        st.tile_m = kTileRows;
-        st.tile_n = kTileRows;
+        run_gram(ctx->prep, ctx->prep, run, ctx->gsc, s, rec);
+        rec->base.main_launches = 1;
+        rec->base.tile_m = kTileRows;
+        rec->base.tile_n = kTileRows;
+    } else {
+        for (int k = 1; k < 4; ++k) {
+            rec->start(k, s);
+            rec->stop(k, s);
+        }
     }
-    st.total_ms = ctx->t_total.ms();
-    st.prep_ms = ctx->t_prep.ms();
-    st.main_ms = ctx->t_main.ms();
-    st.fallback_ms = ctx->t_fallback.ms();
-    ctx->last = st;
+    rec->stop(kRecTotal, s);
 }
 
 }  // namespace bs
+
+bs::RunRecord* bs_context::begin_record()
+{
+    constexpr size_t kRing = 64;
+    if (ring.empty()) {
+        ring.resize(kRing);
+        for (auto& r : ring) r.create();
+        BS_HIP(hipHostMalloc((void**)&h_counts, kRing * sizeof(uint32_t), hipHostMallocDefault));
+        for (size_t k = 0; k < kRing; ++k) ring[k].h_count = h_counts + k;
+    }
+    bs::RunRecord* r = &ring[ring_next];
+    ring_next = (ring_next + 1) % ring.size();
+    ring_used = std::min(ring_used + 1, ring.size());
+    if (r->used) BS_HIP(hipEventSynchronize(r->ev[2 * bs::kRecTotal + 1]));  // never reuse in flight
+    r->base = bs_run_stats{};
+    r->gram = false;
+    r->used = true;
+    return r;
+}

@@ -54,15 +54,32 @@ struct GramScratch {
     uint64_t tiles_key = ~0ull;
     int tiles_n = 0;
     int max_flagged = 0;
-    uint32_t* h_count = nullptr;  // pinned
+    int max_wg = 0;               // persistent Gram work-groups (2 per CU)
+    int skew_sleeps = 0;          // phase offset of the second half of the grid (simple variant)
+    int variant = 0;              // 0: one tile per work-group, 1: software-pipelined, persistent
     void bind(Budget* b) { flags.bind(b); tiles.bind(b); }
 };
 
+// Events and host-visible results of one call, read lazily.
+struct RunRecord {
+    hipEvent_t ev[8] = {};       // total, prep, main, fallback: start/stop pairs
+    uint32_t* h_count = nullptr;  // flagged-pair count (pinned), copied after the call
+    bs_run_stats base{};          // pairs, launches, tile sizes
+    bool gram = false;
+    bool used = false;
+    void create();
+    void destroy();
+    void start(int k, hipStream_t s) { BS_HIP(hipEventRecord(ev[2 * k], s)); }
+    void stop(int k, hipStream_t s) { BS_HIP(hipEventRecord(ev[2 * k + 1], s)); }
+    bs_run_stats read() const;  // synchronises on the last event
+};
+enum { kRecTotal = 0, kRecPrep = 1, kRecMain = 2, kRecFallback = 3 };
+
 // Gram contraction + QCP epilogue + FP64 re-evaluation of flagged pairs over
-// rows of A x cols of B (or the upper triangle of A when triangle).  Returns
-// the number of flagged pairs.  Synchronises the stream at the end.
-uint64_t run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
-                  hipStream_t s, Timer* t_main, Timer* t_fb, int* launches);
+// rows of A x cols of B (or the upper triangle of A when triangle).  Asynchronous:
+// the flagged-pair count is copied to rec->h_count when rec is given.
+void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
+              hipStream_t s, RunRecord* rec);
 
 GramMode mode_of(bs_method m);
 bool is_gram(bs_method m);
@@ -81,8 +98,10 @@ struct bs_context {
     bs::Prepared prep;         // the last ensemble's prepared set
     bs::GramScratch gsc;
     bs::DevBuf ws_xyz, ws_out, ws_misc;
-    bs::Timer t_total, t_prep, t_main, t_fallback;
-    bs_run_stats last{};
+    std::vector<bs::RunRecord> ring;  // the last calls' records
+    size_t ring_next = 0, ring_used = 0;
+    uint32_t* h_counts = nullptr;      // pinned, one per record
+    bs::RunRecord* begin_record();     // next ring slot, recycled
     std::vector<int> tune_variant;  // autotuner: Gram tile variant per atom-count bucket
     int n_cu = 0;
     std::mutex mu;

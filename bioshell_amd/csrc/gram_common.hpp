@@ -1,0 +1,125 @@
+// gram_common.hpp -- pieces shared by the Gram kernels.
+#pragma once
+
+#include "kernels.hpp"
+#include "qcp.hpp"
+
+namespace bs {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kXcd = 8;  // MI355X: work-group ids are dealt to 8 XCDs round-robin
+
+__device__ inline int64_t tri_index(int64_t n, int64_t i, int64_t j)
+{
+    return i * n - i * (i + 1) / 2 + (j - i - 1);
+}
+
+__device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// One atom block of the nine 32x32 accumulators of a wave: rows from the A
+// stage block (32 conformations), columns from the B stage block.
+template <int MODE>
+__device__ __forceinline__ void mfma_block(const uint4* __restrict__ A, const uint4* __restrict__ B,
+                                           int lane, f32x16 (&acc)[9])
+{
+    if constexpr (MODE == kGramBF16x3) {
+        bf16x8 ah[3], al[3], bh[3], bl[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ah[c] = as_bf16x8(A[(2 * c) * 64 + lane]);
+            al[c] = as_bf16x8(A[(2 * c + 1) * 64 + lane]);
+            bh[c] = as_bf16x8(B[(2 * c) * 64 + lane]);
+            bl[c] = as_bf16x8(B[(2 * c + 1) * 64 + lane]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                acc[3 * c + d] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh[d], acc[3 * c + d], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                acc[3 * c + d] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl[d], acc[3 * c + d], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                acc[3 * c + d] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh[d], acc[3 * c + d], 0, 0, 0);
+    } else {
+        float fa[3][8], fb[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint4 a0 = A[c * 128 + 2 * lane], a1 = A[c * 128 + 2 * lane + 1];
+            const uint4 b0 = B[c * 128 + 2 * lane], b1 = B[c * 128 + 2 * lane + 1];
+            fa[c][0] = __uint_as_float(a0.x); fa[c][1] = __uint_as_float(a0.y);
+            fa[c][2] = __uint_as_float(a0.z); fa[c][3] = __uint_as_float(a0.w);
+            fa[c][4] = __uint_as_float(a1.x); fa[c][5] = __uint_as_float(a1.y);
+            fa[c][6] = __uint_as_float(a1.z); fa[c][7] = __uint_as_float(a1.w);
+            fb[c][0] = __uint_as_float(b0.x); fb[c][1] = __uint_as_float(b0.y);
+            fb[c][2] = __uint_as_float(b0.z); fb[c][3] = __uint_as_float(b0.w);
+            fb[c][4] = __uint_as_float(b1.x); fb[c][5] = __uint_as_float(b1.y);
+            fb[c][6] = __uint_as_float(b1.z); fb[c][7] = __uint_as_float(b1.w);
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[c][s], fb[d][s],
+                                                                          acc[3 * c + d], 0, 0, 0);
+    }
+}
+
+// FP64 cRMSD of one pair from centred coordinates, by one lane (used only when
+// the flagged-pair list is full, so that no flagged pair is ever dropped).
+__device__ __noinline__ double pair_rmsd_fp64(const double* __restrict__ x,
+                                              const double* __restrict__ y, int L)
+{
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ga = 0.0, gb = 0.0;
+    for (int k = 0; k < L; ++k) {
+        const double a0 = x[3 * k], a1 = x[3 * k + 1], a2 = x[3 * k + 2];
+        const double b0 = y[3 * k], b1 = y[3 * k + 1], b2 = y[3 * k + 2];
+        m[0] += a0 * b0; m[1] += a0 * b1; m[2] += a0 * b2;
+        m[3] += a1 * b0; m[4] += a1 * b1; m[5] += a1 * b2;
+        m[6] += a2 * b0; m[7] += a2 * b1; m[8] += a2 * b2;
+        ga += a0 * a0 + a1 * a1 + a2 * a2;
+        gb += b0 * b0 + b1 * b1 + b2 * b2;
+    }
+    return qcp_rmsd<double>(m, ga, gb, L);
+}
+
+// Appends flagged pairs of one wave to the re-evaluation list (one atomic per
+// wave); a pair beyond the list's capacity is re-evaluated here in FP64.
+__device__ __forceinline__ void append_flagged(const GramArgs& a, bool flag, int i, int j, int lane)
+{
+    const uint64_t m = __ballot(flag);
+    if (!m) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(a.flag_count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (!flag) return;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+    if ((int)pos < a.max_flagged) {
+        a.flag_pairs[2 * (int64_t)pos] = (uint32_t)i;
+        a.flag_pairs[2 * (int64_t)pos + 1] = (uint32_t)j;
+    } else {
+        const double r = pair_rmsd_fp64(a.centred_a + (int64_t)i * a.L * 3,
+                                        a.centred_b + (int64_t)j * a.L * 3, a.L);
+        const int64_t oi = a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
+        if (a.out_f64)
+            static_cast<double*>(a.out)[oi] = r;
+        else
+            static_cast<float*>(a.out)[oi] = (float)r;
+    }
+}
+
+}  // namespace bs

@@ -66,6 +66,8 @@ struct GramArgs {
     int64_t out_ld;         // dense output: row stride (triangle: ignored)
     void* out;
     int out_f64;
+    const double* centred_a;  // FP64 centred rows (n_rows x L x 3): inline re-evaluation
+    const double* centred_b;  // FP64 centred columns
     uint32_t* flag_pairs;   // (i, j) of pairs to re-evaluate in FP64
     uint32_t* flag_count;
     int max_flagged;
@@ -75,6 +77,9 @@ struct GramArgs {
     double acc_coef;        // 2^-24 * sqrt(accumulations)
     double tau;             // tolerated RMSD error estimate (A)
     double rep_g_max;       // g_i + g_j above which the split's rounding of e may show
+    // launch shape
+    int max_wg;             // resident work-groups (2 per CU)
+    int skew_sleeps;        // s_sleep(127) count delaying ids [max_wg/2, max_wg)
 };
 
 // Builds the logical tile order (XCD-aware super-blocks) on the host.
@@ -82,6 +87,8 @@ struct GramArgs {
 int build_tile_order(int tiles_r, int tiles_c, bool triangle, int super, uint32_t* out);
 
 void launch_gram(const GramArgs& a, GramMode mode, hipStream_t s);
+// The software-pipelined variant (one persistent work-group per CU).
+void launch_gram_pipe(const GramArgs& a, GramMode mode, hipStream_t s);
 void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
 // FP64 re-evaluation of flagged pairs from centred coordinates.

@@ -16,36 +16,19 @@
 // through LDS.  Work-group ids are dealt to the 8 XCDs round-robin, so the
 // logical tile is taken from a contiguous per-XCD slice of a super-blocked
 // order: each XCD's L2 keeps re-using the same row and column panels.
-#include "kernels.hpp"
-#include "qcp.hpp"
+#include "gram_common.hpp"
+
+#include <algorithm>
 
 namespace bs {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kXcd = 8;
-
-__device__ inline int64_t tri_index(int64_t n, int64_t i, int64_t j)
-{
-    return i * n - i * (i + 1) / 2 + (j - i - 1);
-}
-
-__device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
-
+// One 64 x 64 tile of pairs: K loop over atom blocks, then the epilogue.
 template <int MODE>
-__global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
+__device__ __forceinline__ void gram_tile(const GramArgs& a, int tr, int tc,
+                                          uint4 (&lds)[2][4][kPanelBlockU4])
 {
-    __shared__ uint4 lds[2][4][kPanelBlockU4];  // 2 stages x (A0, A1, B0, B1) x 6 KiB
-
-    const int wg = blockIdx.x;
-    const int per = (a.n_tiles + kXcd - 1) / kXcd;
-    const int t = (wg % kXcd) * per + (wg / kXcd);
-    if (t >= a.n_tiles) return;
-    const uint32_t code = a.tiles[t];
-    const int tr = (int)(code >> 16), tc = (int)(code & 0xFFFFu);
-
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
 
@@ -98,57 +81,7 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
         }
         const uint4* A = lds[cur][wr];
         const uint4* B = lds[cur][2 + wc];
-        if constexpr (MODE == kGramBF16x3) {
-            bf16x8 ah[3], al[3], bh[3], bl[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ah[c] = as_bf16x8(A[(2 * c) * 64 + lane]);
-                al[c] = as_bf16x8(A[(2 * c + 1) * 64 + lane]);
-                bh[c] = as_bf16x8(B[(2 * c) * 64 + lane]);
-                bl[c] = as_bf16x8(B[(2 * c + 1) * 64 + lane]);
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    acc[3 * c + d] =
-                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh[d], acc[3 * c + d], 0, 0, 0);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    acc[3 * c + d] =
-                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl[d], acc[3 * c + d], 0, 0, 0);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    acc[3 * c + d] =
-                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh[d], acc[3 * c + d], 0, 0, 0);
-        } else {
-            float fa[3][8], fb[3][8];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const uint4 a0 = A[c * 128 + 2 * lane], a1 = A[c * 128 + 2 * lane + 1];
-                const uint4 b0 = B[c * 128 + 2 * lane], b1 = B[c * 128 + 2 * lane + 1];
-                fa[c][0] = __uint_as_float(a0.x); fa[c][1] = __uint_as_float(a0.y);
-                fa[c][2] = __uint_as_float(a0.z); fa[c][3] = __uint_as_float(a0.w);
-                fa[c][4] = __uint_as_float(a1.x); fa[c][5] = __uint_as_float(a1.y);
-                fa[c][6] = __uint_as_float(a1.z); fa[c][7] = __uint_as_float(a1.w);
-                fb[c][0] = __uint_as_float(b0.x); fb[c][1] = __uint_as_float(b0.y);
-                fb[c][2] = __uint_as_float(b0.z); fb[c][3] = __uint_as_float(b0.w);
-                fb[c][4] = __uint_as_float(b1.x); fb[c][5] = __uint_as_float(b1.y);
-                fb[c][6] = __uint_as_float(b1.z); fb[c][7] = __uint_as_float(b1.w);
-            }
-#pragma unroll
-            for (int s = 0; s < 8; ++s)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                            fa[c][s], fb[d][s], acc[3 * c + d], 0, 0, 0);
-        }
+        mfma_block<MODE>(A, B, lane, acc);
         if (more) {
             BS_LSTORE(cur ^ 1)
         }
@@ -187,7 +120,6 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
     const double Ld = (double)a.L;
     const double invL = 1.0 / Ld;
     const double tau2L = a.tau * a.tau * Ld;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
 #define BS_STASH(BASE)                                                   \
     _Pragma("unroll") for (int s_ = 0; s_ < 4; ++s_)                     \
@@ -238,24 +170,31 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
                     else
                         static_cast<float*>(a.out)[oi] = sqrtf((float)(o[h].rmsd2_L * invL));
                 }
-                const uint64_t m = __ballot(flag);
-                if (m) {
-                    const int leader = __ffsll((unsigned long long)m) - 1;
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(a.flag_count, (uint32_t)__popcll(m));
-                    base = __shfl(base, leader);
-                    if (flag) {
-                        const uint32_t pos = base + (uint32_t)__popcll(m & lt_mask);
-                        if ((int)pos < a.max_flagged) {
-                            a.flag_pairs[2 * (int64_t)pos] = (uint32_t)i;
-                            a.flag_pairs[2 * (int64_t)pos + 1] = (uint32_t)j;
-                        }
-                    }
-                }
+                append_flagged(a, flag, i, j, lane);
             }
         }
     }
 #undef BS_STASH
+}
+
+// One work-group per 64 x 64 tile.  Work-group ids are dealt to the 8 XCDs
+// round-robin, so the logical tile comes from a contiguous per-XCD slice of
+// the super-blocked order.  Two work-groups share each CU; the ones in the
+// second resident batch (ids [n_cu, 2 n_cu)) start about half a tile late, so
+// that on every SIMD one work-group's MFMA phase overlaps the other's FP64
+// epilogue (VALU).  Tiles cost the same, so refilled slots keep the offset.
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
+{
+    __shared__ uint4 lds[2][4][kPanelBlockU4];  // 2 stages x (A0, A1, B0, B1) x 6 KiB
+    const int wg = blockIdx.x;
+    const int per = (a.n_tiles + kXcd - 1) / kXcd;
+    const int t = (wg % kXcd) * per + (wg / kXcd);
+    if (t >= a.n_tiles) return;
+    if (a.skew_sleeps > 0 && wg >= a.max_wg / 2 && wg < a.max_wg)
+        for (int k = 0; k < a.skew_sleeps; ++k) __builtin_amdgcn_s_sleep(127);
+    const uint32_t code = a.tiles[t];
+    gram_tile<MODE>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n)

@@ -275,21 +275,19 @@ BS_HD double dpoly_at(const ShiftedPoly& s, double m)
 // (P, P', P''/2, P'''/6, 1) are >= 0.  Newton from an upper bound of a
 // real-rooted polynomial converges monotonically to the largest root;
 // otherwise the start is the upper bound H (lambda <= E0).
+// Evaluated without branches (so it can share a basic block with MFMAs):
+// when p1 <= 0 the candidate is inf or NaN and every test below fails.
 BS_HD double shifted_start(const ShiftedPoly& s)
 {
     const double Hp = s.H > 0.0 ? s.H : 0.0;
-    double mu = Hp;
-    if (s.p1 > 0.0) {
-        const double m = -newton_div(s.p0, s.p1);
-        if (m >= 0.0 && m < Hp) {
-            const double t0 = poly_at(s, m);
-            const double t1d = dpoly_at(s, m);
-            const double t2d = (6.0 * m + 3.0 * s.p3) * m + s.p2;
-            const double t3d = 4.0 * m + s.p3;
-            if (t0 >= 0.0 && t1d >= 0.0 && t2d >= 0.0 && t3d >= 0.0) mu = m;
-        }
-    }
-    return mu;
+    const double m = -newton_div(s.p0, s.p1);
+    const double t0 = poly_at(s, m);
+    const double t1d = dpoly_at(s, m);
+    const double t2d = (6.0 * m + 3.0 * s.p3) * m + s.p2;
+    const double t3d = 4.0 * m + s.p3;
+    const bool use = (s.p1 > 0.0) & (m >= 0.0) & (m < Hp) & (t0 >= 0.0) & (t1d >= 0.0) &
+                     (t2d >= 0.0) & (t3d >= 0.0);
+    return use ? m : Hp;
 }
 
 // Newton state of one pair.

@@ -98,6 +98,10 @@ bs_status bs_context_synchronize(bs_context* ctx);
 /* The hipStream_t (as void*) on which the context launches its kernels. */
 void* bs_context_stream(bs_context* ctx);
 bs_status bs_context_last_stats(const bs_context* ctx, bs_run_stats* out);
+/* Stats of up to max_calls most recent calls (oldest first; the context keeps
+ * the last 64); *n_out receives how many were written.  Synchronises on them. */
+bs_status bs_context_history(const bs_context* ctx, bs_run_stats* out, size_t max_calls,
+                             size_t* n_out);
 /* Device bytes currently allocated by the context (workspaces included). */
 size_t bs_context_bytes_in_use(const bs_context* ctx);
 
@@ -129,8 +133,10 @@ bs_status bs_crmsd_all_vs_all(bs_context* ctx, const double* xyz, size_t n_conf,
                               size_t n_atoms, bs_method method, void* out_upper, int out_f64);
 
 /* The same on device-resident buffers (d_xyz and d_out_upper are device
- * pointers, e.g. from hipMalloc); the call is asynchronous on the context's
- * stream.  Used when the ensemble already lives in HBM. */
+ * pointers, e.g. from hipMalloc).  The call is asynchronous on the context's
+ * stream (see bs_context_stream / bs_context_synchronize); d_xyz must stay
+ * valid and d_out_upper unread until the stream reaches the call's end.  Used
+ * when the ensemble already lives in HBM. */
 bs_status bs_crmsd_all_vs_all_device(bs_context* ctx, const double* d_xyz, size_t n_conf,
                                      size_t n_atoms, bs_method method, void* d_out_upper,
                                      int out_f64);

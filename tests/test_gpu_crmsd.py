@@ -178,5 +178,24 @@ def test_device_pointer_entry(gpu_ctx):
     dout = torch.empty(len(want), dtype=torch.float32, device="cuda:0")
     torch.cuda.synchronize()
     B.crmsd_all_vs_all_device(dx.data_ptr(), 257, 90, dout.data_ptr(), B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    gpu_ctx.synchronize()  # the device entry point is asynchronous on the context's stream
     got = dout.cpu().numpy().astype(np.float64)
+    st = gpu_ctx.last_stats()
+    assert st["pairs"] == 257 * 256 // 2 and st["main_ms"] > 0.0
+    assert len(gpu_ctx.history(3)) == 3
     assert np.abs(got - want).max() <= 1e-4
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_gram_kernel_variants(variant, monkeypatch):
+    """Both Gram kernels (one tile per work-group; persistent software-pipelined)."""
+    monkeypatch.setenv("BIOSHELL_GRAM_VARIANT", str(variant))
+    x = B.generate_ensemble(8, 700, 120, sigma=(0.5, 4.0), near_fraction=0.05)
+    want = O.all_vs_all(x, nthreads=8)
+    with B.Context(0) as ctx:
+        for method in (B.Method.BF16X3_GRAM, B.Method.FP32_GRAM):
+            for f64 in (False, True):
+                got = B.crmsd_all_vs_all(x, method, out_f64=f64, ctx=ctx).upper.astype(np.float64)
+                err = np.abs(got - want)
+                report(f"variant {variant} {method.name} f64={f64}", err)
+                assert err.max() <= TOL[method]
