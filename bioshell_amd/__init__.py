@@ -6,8 +6,11 @@ Hand-written HIP kernels for gfx950 behind the C ABI of
 from .crmsd import (  # noqa: F401
     BioshellGpuError,
     Context,
+    Library,
     Method,
     RmsdMatrix,
+    Server,
+    Ticket,
     Rototranslation,
     check,
     crmsd,

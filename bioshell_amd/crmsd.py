@@ -276,3 +276,126 @@ def generate_ensemble(seed: int, n_conf: int, n_atoms: int, sigma=(1.5, 1.5), ne
     out = np.empty((n_conf, n_atoms, 3), dtype=np.float64)
     check(_native.lib().bs_generate_ensemble(seed, n_conf, n_atoms, lo, hi, near_fraction, near_sigma, _dp(out)))
     return out
+
+
+# --------------------------------------------------------------- libraries
+class Library:
+    """Device-resident conformations of one atom count (bs_library)."""
+
+    def __init__(self, xyz, ctx: Context | None = None):
+        x = _ensemble(xyz)
+        self.ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        check(_native.lib().bs_library_create(self.ctx.handle, _dp(x), x.shape[0], x.shape[1], ctypes.byref(h)))
+        self._h = h
+        self.n, self.n_atoms = x.shape[0], x.shape[1]
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("library is closed")
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            _native.lib().bs_library_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _queries(self, q) -> np.ndarray:
+        q = _f64(q)
+        if q.ndim == 2:
+            q = q[None]
+        if q.ndim != 3 or q.shape[1:] != (self.n_atoms, 3):
+            raise ValueError(f"queries must be (n_q, {self.n_atoms}, 3), got {q.shape}")
+        return np.ascontiguousarray(q)
+
+    def query_all(self, q) -> np.ndarray:
+        """cRMSD of every query against every library member: (n_q, n) f32."""
+        q = self._queries(q)
+        out = np.empty((q.shape[0], self.n), dtype=np.float32)
+        check(_native.lib().bs_query_all(self.ctx.handle, self.handle, _dp(q), q.shape[0],
+                                         out.ctypes.data_as(_native.c_fp)))
+        return out
+
+    def query_topk(self, q, k: int = 100) -> tuple[np.ndarray, np.ndarray]:
+        """Per query, the k members of lowest cRMSD: (idx u32, rmsd f32), each (n_q, k)."""
+        q = self._queries(q)
+        idx = np.empty((q.shape[0], k), dtype=np.uint32)
+        val = np.empty((q.shape[0], k), dtype=np.float32)
+        check(_native.lib().bs_query_topk(self.ctx.handle, self.handle, _dp(q), q.shape[0], k,
+                                          idx.ctypes.data_as(_native.c_u32p), val.ctypes.data_as(_native.c_fp)))
+        return idx, val
+
+
+# --------------------------------------------------------------- serving
+class Ticket:
+    """A submitted request; keeps its buffers alive until it completes."""
+
+    def __init__(self, server: "Server", tid: int, q: np.ndarray, idx: np.ndarray, val: np.ndarray):
+        self.server, self.id, self._q, self.idx, self.rmsd = server, tid, q, idx, val
+
+    def poll(self) -> bool:
+        st = _native.lib().bs_server_poll(self.server.handle, self.id)
+        if st == -6:
+            return False
+        check(st)
+        return True
+
+    def wait(self) -> tuple[np.ndarray, np.ndarray]:
+        check(_native.lib().bs_server_wait(self.server.handle, self.id))
+        return self.idx, self.rmsd
+
+    def cancel(self) -> None:
+        check(_native.lib().bs_server_cancel(self.server.handle, self.id))
+
+    def latency_ms(self) -> float:
+        ms = ctypes.c_double()
+        check(_native.lib().bs_server_latency_ms(self.server.handle, self.id, ctypes.byref(ms)))
+        return ms.value
+
+
+class Server:
+    """Continuously batching query server over device-resident libraries (bs_server)."""
+
+    def __init__(self, libraries: list[Library], hbm_budget_bytes: int = 0, max_inflight: int = 4,
+                 ctx: Context | None = None):
+        self.ctx = ctx or libraries[0].ctx
+        self.libraries = list(libraries)
+        arr = (ctypes.c_void_p * len(libraries))(*[l.handle.value for l in libraries])
+        h = ctypes.c_void_p()
+        check(_native.lib().bs_server_create(self.ctx.handle, arr, len(libraries), hbm_budget_bytes,
+                                             max_inflight, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("server is closed")
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            _native.lib().bs_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, lib_index: int, q, k: int = 100) -> Ticket:
+        q = self.libraries[lib_index]._queries(q)
+        idx = np.empty((q.shape[0], k), dtype=np.uint32)
+        val = np.empty((q.shape[0], k), dtype=np.float32)
+        t = ctypes.c_uint64()
+        check(_native.lib().bs_server_submit(self.handle, lib_index, _dp(q), q.shape[0], k,
+                                             idx.ctypes.data_as(_native.c_u32p),
+                                             val.ctypes.data_as(_native.c_fp), ctypes.byref(t)))
+        return Ticket(self, t.value, q, idx, val)

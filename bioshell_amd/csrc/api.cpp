@@ -92,6 +92,7 @@ void bs_context_destroy(bs_context* ctx)
         DeviceGuard dg(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         ctx->prep.reset();
+        ctx->qws.reset();
         for (auto& r : ctx->ring) r.destroy();
         ctx->ring.clear();
         if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);

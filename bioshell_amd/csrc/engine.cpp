@@ -39,13 +39,13 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
                                    "operand panels");
     double* consts = static_cast<double*>(P.consts.ensure((size_t)kNumConsts * P.n_pad * 8, "constants"));
 
-    launch_centre(d_xyz, n, L, centred, s);
     if (fixed_ref) {
+        launch_centre(d_xyz, n, L, centred, s);
         BS_HIP(hipMemcpyAsync(ref, fixed_ref, (size_t)(3 * L + 6) * 8, hipMemcpyDeviceToDevice, s));
         launch_fit(centred, n, L, ref, rot, s);
     } else {
-        launch_ref_from_first(centred, L, ref, s);
-        launch_fit(centred, n, L, ref, rot, s);
+        // fit onto conformation 0, one Procrustes-mean step, fit onto the mean
+        launch_centre_fit0(d_xyz, n, L, centred, rot, s);
         launch_mean(centred, rot, n, L, partial, ref, s);
         launch_fit(centred, n, L, ref, rot, s);
     }

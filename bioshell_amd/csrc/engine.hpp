@@ -81,6 +81,18 @@ enum { kRecTotal = 0, kRecPrep = 1, kRecMain = 2, kRecFallback = 3 };
 void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
               hipStream_t s, RunRecord* rec);
 
+// Workspace of one in-flight query batch (query prep, score matrix, top-k).
+struct QueryWorkspace {
+    Prepared prep;
+    DevBuf xyz, scores, topk;
+    GramScratch gsc;
+    void bind(Budget* b);
+    void reset();
+};
+
+// Device bytes a query batch of n_q against n_lib conformations of L atoms needs.
+size_t query_workspace_bytes(int n_q, int n_lib, int L, int k);
+
 GramMode mode_of(bs_method m);
 bool is_gram(bs_method m);
 
@@ -98,6 +110,7 @@ struct bs_context {
     bs::Prepared prep;         // the last ensemble's prepared set
     bs::GramScratch gsc;
     bs::DevBuf ws_xyz, ws_out, ws_misc;
+    bs::QueryWorkspace qws;    // synchronous query calls
     std::vector<bs::RunRecord> ring;  // the last calls' records
     size_t ring_next = 0, ring_used = 0;
     uint32_t* h_counts = nullptr;      // pinned, one per record

@@ -38,6 +38,9 @@ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // Centre every conformation (sum, divide, subtract).
 void launch_centre(const double* xyz, int n, int L, double* centred, hipStream_t s);
+// Centring fused with the first reference fit (onto conformation 0).
+void launch_centre_fit0(const double* xyz, int n, int L, double* centred, double* rot,
+                        hipStream_t s);
 // Reference fit: rotation of each centred conformation onto ref (L x 3).
 void launch_fit(const double* centred, int n, int L, const double* ref, double* rot,
                 hipStream_t s);

@@ -46,6 +46,50 @@ __global__ __launch_bounds__(256) void centre_kernel(const double* __restrict__ 
     }
 }
 
+// Centre every conformation and, in the same pass, reduce its covariance with
+// conformation 0 as it comes (uncentred): sum (x_k - c) x0_k^T equals the
+// covariance with the centred x0 because sum (x_k - c) = 0.  cov gets 9
+// doubles per conformation (the first reference fit).
+__global__ __launch_bounds__(256) void centre_cov0_kernel(const double* __restrict__ xyz, int n,
+                                                          int L, double* __restrict__ out,
+                                                          double* __restrict__ cov)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double* p = xyz + (int64_t)i * L * 3;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = lane; k < L; k += 64) {
+        s0 += p[3 * k];
+        s1 += p[3 * k + 1];
+        s2 += p[3 * k + 2];
+    }
+    s0 = wave_sum(s0) / L;
+    s1 = wave_sum(s1) / L;
+    s2 = wave_sum(s2) / L;
+    double* q = out + (int64_t)i * L * 3;
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = lane; k < L; k += 64) {
+        const double b0 = p[3 * k] - s0, b1 = p[3 * k + 1] - s1, b2 = p[3 * k + 2] - s2;
+        q[3 * k] = b0;
+        q[3 * k + 1] = b1;
+        q[3 * k + 2] = b2;
+        const double a0 = xyz[3 * k], a1 = xyz[3 * k + 1], a2 = xyz[3 * k + 2];
+        m[0] += b0 * a0; m[1] += b0 * a1; m[2] += b0 * a2;
+        m[3] += b1 * a0; m[4] += b1 * a1; m[5] += b1 * a2;
+        m[6] += b2 * a0; m[7] += b2 * a1; m[8] += b2 * a2;
+    }
+#pragma unroll
+    for (int qq = 0; qq < 9; ++qq) m[qq] = wave_sum(m[qq]);
+    if (lane < 9) {
+        double v = m[0];
+#pragma unroll
+        for (int qq = 1; qq < 9; ++qq)
+            if (lane == qq) v = m[qq];
+        cov[(int64_t)i * 9 + lane] = v;
+    }
+}
+
 // Rotation that moves b onto a, from M = sum b a^T (row-major).
 __device__ void rotation_from_cov(const double* M, double* R, double* lambda)
 {
@@ -350,6 +394,13 @@ __global__ __launch_bounds__(64) void superpose_kernel(const double* __restrict_
 void launch_centre(const double* xyz, int n, int L, double* centred, hipStream_t s)
 {
     centre_kernel<<<(n + 3) / 4, 256, 0, s>>>(xyz, n, L, centred);
+}
+
+void launch_centre_fit0(const double* xyz, int n, int L, double* centred, double* rot,
+                        hipStream_t s)
+{
+    centre_cov0_kernel<<<(n + 3) / 4, 256, 0, s>>>(xyz, n, L, centred, rot);
+    fit_rot_kernel<<<(n + 63) / 64, 64, 0, s>>>(rot, n);
 }
 
 void launch_fit(const double* centred, int n, int L, const double* ref, double* rot, hipStream_t s)

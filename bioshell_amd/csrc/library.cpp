@@ -1,0 +1,173 @@
+// library.cpp -- device-resident conformation libraries and query-vs-library
+// cRMSD (full matrix or per-query top-k).
+//
+// A library is prepared once (centring, fit onto its Procrustes-mean
+// reference, operand panels, constants) and kept in HBM.  Queries are centred
+// and fitted onto the library's reference, packed the same way, and scored by
+// the rectangular Gram kernel (rows = queries, columns = library) with the
+// FP64 QCP epilogue; top-k then selects per query.
+#include <algorithm>
+#include <cstring>
+
+#include "library.hpp"
+
+using namespace bs;
+
+namespace bs {
+
+size_t query_workspace_bytes(int n_q, int n_lib, int L, int k)
+{
+    return prepared_bytes(n_q, L) + (size_t)n_q * L * 3 * 8 + (size_t)n_q * n_lib * 4 +
+           (size_t)n_q * k * 8 + 4096;
+}
+
+void QueryWorkspace::bind(Budget* b)
+{
+    prep.bind(b);
+    xyz.bind(b);
+    scores.bind(b);
+    topk.bind(b);
+    gsc.bind(b);
+}
+
+void QueryWorkspace::reset()
+{
+    prep.reset();
+    xyz.reset();
+    scores.reset();
+    topk.reset();
+    gsc.flags.reset();
+    gsc.tiles.reset();
+    gsc.tiles_key = ~0ull;
+}
+
+// Enqueues the whole query path on stream s: H2D of the queries (from
+// q_host, which must stay valid until the stream passes the copy), prep
+// against the library's reference, Gram + epilogue + FP64 re-evaluation into
+// ws.scores (n_q x n_lib), and, when k > 0, top-k into ws.topk.
+void enqueue_query(bs_library* lib, QueryWorkspace& ws, const double* q_host, int n_q, int k,
+                   hipStream_t s, RunRecord* rec)
+{
+    const int L = lib->P.L, n_lib = lib->P.n;
+    double* dq = static_cast<double*>(ws.xyz.ensure((size_t)n_q * L * 3 * 8, "query coordinates"));
+    float* scores =
+        static_cast<float*>(ws.scores.ensure((size_t)n_q * n_lib * 4, "query scores"));
+    BS_HIP(hipMemcpyAsync(dq, q_host, (size_t)n_q * L * 3 * 8, hipMemcpyHostToDevice, s));
+    if (rec) rec->start(kRecTotal, s);
+    if (rec) rec->start(kRecPrep, s);
+    prepare(ws.prep, dq, n_q, L, lib->P.mode, lib->P.ref.as<double>(), s);
+    if (rec) rec->stop(kRecPrep, s);
+    GramRun run;
+    run.triangle = false;
+    run.d_out = scores;
+    run.out_f64 = false;
+    run.out_ld = n_lib;
+    ws.gsc.max_wg = lib->ctx->gsc.max_wg;
+    ws.gsc.variant = lib->ctx->gsc.variant;
+    run_gram(ws.prep, lib->P, run, ws.gsc, s, rec);
+    if (k > 0) {
+        uint32_t* idx = static_cast<uint32_t*>(ws.topk.ensure((size_t)n_q * k * 8, "top-k"));
+        float* val = reinterpret_cast<float*>(idx + (size_t)n_q * k);
+        launch_topk(scores, n_q, n_lib, k, idx, val, s);
+        BS_HIP(hipGetLastError());
+    }
+    if (rec) rec->stop(kRecTotal, s);
+    if (rec) {
+        rec->base.pairs = (uint64_t)n_q * n_lib;
+        rec->base.main_launches = 1;
+        rec->base.tile_m = 64;
+        rec->base.tile_n = 64;
+    }
+}
+
+}  // namespace bs
+
+extern "C" {
+
+bs_status bs_library_create(bs_context* ctx, const double* xyz, size_t n_conf, size_t n_atoms,
+                            bs_library** out)
+{
+    return guard([&] {
+        BS_REQUIRE(ctx != nullptr && out != nullptr, "null pointer argument");
+        BS_REQUIRE(xyz != nullptr, "xyz is null");
+        BS_REQUIRE(n_conf >= 1 && n_atoms >= 1, "a library needs >= 1 conformation of >= 1 atom");
+        BS_REQUIRE(n_conf < (1u << 21) && n_atoms <= (1u << 20), "library too large");
+        *out = nullptr;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        BS_HIP(hipSetDevice(ctx->device));
+        auto lib = std::make_unique<bs_library>();
+        lib->ctx = ctx;
+        lib->P.bind(&ctx->budget);
+        lib->staging.bind(&ctx->budget);
+        hipStream_t s = ctx->stream;
+        const size_t bytes = n_conf * n_atoms * 3 * 8;
+        double* d = static_cast<double*>(lib->staging.ensure(bytes, "library coordinates"));
+        BS_HIP(hipMemcpyAsync(d, xyz, bytes, hipMemcpyHostToDevice, s));
+        prepare(lib->P, d, (int)n_conf, (int)n_atoms, kGramBF16x3, nullptr, s);
+        BS_HIP(hipStreamSynchronize(s));
+        lib->staging.reset();  // only the prepared set stays resident
+        lib->P.partial.reset();
+        *out = lib.release();
+    });
+}
+
+void bs_library_destroy(bs_library* lib)
+{
+    if (!lib) return;
+    (void)guard([&] {
+        BS_HIP(hipSetDevice(lib->ctx->device));
+        lib->P.reset();
+    });
+    delete lib;
+}
+
+size_t bs_library_size(const bs_library* lib) { return lib ? (size_t)lib->P.n : 0; }
+size_t bs_library_atoms(const bs_library* lib) { return lib ? (size_t)lib->P.L : 0; }
+
+static void check_query(bs_context* ctx, bs_library* lib, const double* q, size_t n_q)
+{
+    BS_REQUIRE(ctx != nullptr && lib != nullptr, "null context or library");
+    BS_REQUIRE(lib->ctx == ctx, "library belongs to another context");
+    BS_REQUIRE(n_q >= 1 && n_q < (1u << 21), "n_q must be in [1, 2^21)");
+    BS_REQUIRE(q != nullptr, "query coordinates are null");
+}
+
+bs_status bs_query_all(bs_context* ctx, bs_library* lib, const double* q_xyz, size_t n_q,
+                       float* out)
+{
+    return guard([&] {
+        check_query(ctx, lib, q_xyz, n_q);
+        BS_REQUIRE(out != nullptr, "out is null");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        BS_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        ctx->qws.bind(&ctx->budget);
+        enqueue_query(lib, ctx->qws, q_xyz, (int)n_q, 0, s, ctx->begin_record());
+        BS_HIP(hipMemcpyAsync(out, ctx->qws.scores.get(), n_q * lib->P.n * 4,
+                              hipMemcpyDeviceToHost, s));
+        BS_HIP(hipStreamSynchronize(s));
+    });
+}
+
+bs_status bs_query_topk(bs_context* ctx, bs_library* lib, const double* q_xyz, size_t n_q,
+                        size_t k, uint32_t* idx, float* rmsd)
+{
+    return guard([&] {
+        check_query(ctx, lib, q_xyz, n_q);
+        BS_REQUIRE(idx != nullptr && rmsd != nullptr, "null output");
+        BS_REQUIRE(k >= 1 && k <= 1024 && k <= (size_t)lib->P.n,
+                   "k must be in [1, min(1024, library size)]");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        BS_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        ctx->qws.bind(&ctx->budget);
+        enqueue_query(lib, ctx->qws, q_xyz, (int)n_q, (int)k, s, ctx->begin_record());
+        const uint32_t* d_idx = ctx->qws.topk.as<uint32_t>();
+        const float* d_val = reinterpret_cast<const float*>(d_idx + n_q * k);
+        BS_HIP(hipMemcpyAsync(idx, d_idx, n_q * k * 4, hipMemcpyDeviceToHost, s));
+        BS_HIP(hipMemcpyAsync(rmsd, d_val, n_q * k * 4, hipMemcpyDeviceToHost, s));
+        BS_HIP(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"
