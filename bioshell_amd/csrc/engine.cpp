@@ -99,16 +99,17 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
         sc.tiles_key = key;
         sc.tiles_n = nt;
     }
-    // The list holds up to 4M pairs; flagged pairs beyond that are re-evaluated
-    // in FP64 inside the Gram kernel, so nothing is dropped and no host check
-    // is needed.
-    const uint64_t pairs = run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * B.n;
-    const int want = (int)std::max<uint64_t>(4096, std::min<uint64_t>(pairs, 1u << 22));
-    sc.max_flagged = std::max(sc.max_flagged, want);
+    // Flagged outputs are marked in a bitmap indexed by output position (one
+    // bit per pair: 6.25 MB for 50M pairs), so there is no capacity limit and
+    // no host round trip; the re-evaluation kernel scans it after the Gram
+    // kernel.  The word after the bitmap counts the re-evaluated pairs.
+    const uint64_t n_out =
+        run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * (uint64_t)run.out_ld;
+    const uint64_t n_words = ((n_out + 31) / 32 + 3) / 4 * 4;
     uint32_t* flags = static_cast<uint32_t*>(
-        sc.flags.ensure((size_t)(2 * (size_t)sc.max_flagged + 64) * 4, "flagged pairs"));
-    uint32_t* count = flags + 2 * (size_t)sc.max_flagged;
-    launch_zero_u32(count, 1, s);
+        sc.flags.ensure((size_t)(n_words + 4) * 4, "flag bitmap"));
+    uint32_t* count = flags + n_words;
+    launch_zero_words(flags, (int64_t)n_words + 4, s);
 
     GramArgs g{};
     g.panels_a = A.panels.get();
@@ -127,11 +128,7 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     g.out_ld = run.out_ld;
     g.out = run.d_out;
     g.out_f64 = run.out_f64 ? 1 : 0;
-    g.centred_a = A.centred.as<double>();
-    g.centred_b = B.centred.as<double>();
-    g.flag_pairs = flags;
-    g.flag_count = count;
-    g.max_flagged = sc.max_flagged;
+    g.flag_bits = flags;
     // Error model of the residual Gram: bf16 hi/lo represents e to 2^-16
     // relative (the dropped lo*lo term is bounded by 2^-16 sqrt(g_i g_j));
     // FP32 accumulation adds ~2^-24 sqrt(#accumulations) sqrt(g_i g_j).
@@ -164,8 +161,8 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     if (rec) rec->stop(kRecMain, s);
     BS_HIP(hipGetLastError());
     if (rec) rec->start(kRecFallback, s);
-    launch_fallback(A.centred.as<double>(), B.centred.as<double>(), A.L, flags, count,
-                    sc.max_flagged, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, s);
+    launch_fallback(A.centred.as<double>(), B.centred.as<double>(), A.L, flags, (int64_t)n_out,
+                    g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, s);
     if (rec) rec->stop(kRecFallback, s);
     BS_HIP(hipGetLastError());
     if (rec && rec->h_count)

@@ -53,7 +53,6 @@ struct GramScratch {
     DevBuf flags, tiles;
     uint64_t tiles_key = ~0ull;
     int tiles_n = 0;
-    int max_flagged = 0;
     int max_wg = 0;               // persistent Gram work-groups (2 per CU)
     int skew_sleeps = 0;          // phase offset of the second half of the grid (simple variant)
     int variant = 0;              // 0: one tile per work-group, 1: software-pipelined, persistent

@@ -77,49 +77,10 @@ __device__ __forceinline__ void mfma_block(const uint4* __restrict__ A, const ui
     }
 }
 
-// FP64 cRMSD of one pair from centred coordinates, by one lane (used only when
-// the flagged-pair list is full, so that no flagged pair is ever dropped).
-__device__ __noinline__ double pair_rmsd_fp64(const double* __restrict__ x,
-                                              const double* __restrict__ y, int L)
+// Marks output o for FP64 re-evaluation.
+__device__ __forceinline__ void flag_output(const GramArgs& a, bool flag, int64_t o)
 {
-    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ga = 0.0, gb = 0.0;
-    for (int k = 0; k < L; ++k) {
-        const double a0 = x[3 * k], a1 = x[3 * k + 1], a2 = x[3 * k + 2];
-        const double b0 = y[3 * k], b1 = y[3 * k + 1], b2 = y[3 * k + 2];
-        m[0] += a0 * b0; m[1] += a0 * b1; m[2] += a0 * b2;
-        m[3] += a1 * b0; m[4] += a1 * b1; m[5] += a1 * b2;
-        m[6] += a2 * b0; m[7] += a2 * b1; m[8] += a2 * b2;
-        ga += a0 * a0 + a1 * a1 + a2 * a2;
-        gb += b0 * b0 + b1 * b1 + b2 * b2;
-    }
-    return qcp_rmsd<double>(m, ga, gb, L);
-}
-
-// Appends flagged pairs of one wave to the re-evaluation list (one atomic per
-// wave); a pair beyond the list's capacity is re-evaluated here in FP64.
-__device__ __forceinline__ void append_flagged(const GramArgs& a, bool flag, int i, int j, int lane)
-{
-    const uint64_t m = __ballot(flag);
-    if (!m) return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(a.flag_count, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (!flag) return;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t pos = base + (uint32_t)__popcll(m & lt);
-    if ((int)pos < a.max_flagged) {
-        a.flag_pairs[2 * (int64_t)pos] = (uint32_t)i;
-        a.flag_pairs[2 * (int64_t)pos + 1] = (uint32_t)j;
-    } else {
-        const double r = pair_rmsd_fp64(a.centred_a + (int64_t)i * a.L * 3,
-                                        a.centred_b + (int64_t)j * a.L * 3, a.L);
-        const int64_t oi = a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
-        if (a.out_f64)
-            static_cast<double*>(a.out)[oi] = r;
-        else
-            static_cast<float*>(a.out)[oi] = (float)r;
-    }
+    if (flag) atomicOr(a.flag_bits + (o >> 5), 1u << (unsigned)(o & 31));
 }
 
 }  // namespace bs

@@ -1,0 +1,194 @@
+// gram_tile.hpp -- one tile of the Gram kernels: K loop and FP64 epilogue.
+#pragma once
+
+#include "gram_common.hpp"
+
+namespace bs {
+
+// The FP64 QCP epilogue of a TILE x TILE tile whose accumulators the calling
+// waves hold (wave w owns rows 32 (w/2) .. +31 and columns 32 (w%2) .. +31).
+// LDS layout (bytes) at `lds`, which the caller no longer needs:
+//   [0, 2 TILE 96)        constants of the tile's rows and columns, 12 doubles each
+//   then, per wave        4 accumulator slots x 9 values x 64 lanes (f32)
+// Accumulators go through LDS in 4 rounds of 4 slots so that one copy of the
+// FP64 solve serves all 16 pairs a lane owns.
+template <int TILE>
+__device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x16 (&acc)[9],
+                                         char* lds, int nthreads)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    constexpr int kConstBytes = 2 * TILE * 12 * 8;
+    double* sconst = reinterpret_cast<double*>(lds);
+    float* sacc = reinterpret_cast<float*>(lds + kConstBytes) + wv * (4 * 9 * 64);
+    for (int v = tid; v < kNumConsts * 2 * TILE; v += nthreads) {
+        const int q = v / (2 * TILE), r = v % (2 * TILE);
+        const bool is_row = r < TILE;
+        const int conf = is_row ? tr * TILE + r : tc * TILE + (r - TILE);
+        const double val = is_row ? (conf < a.n_rows ? a.consts_a[(int64_t)q * a.lda + conf] : 0.0)
+                                  : (conf < a.n_cols ? a.consts_b[(int64_t)q * a.ldb + conf] : 0.0);
+        sconst[r * 12 + q] = val;
+    }
+    __syncthreads();
+
+    const int jl = wc * 32 + (lane & 31);  // column within the tile
+    const int j = tc * TILE + jl;
+    const bool jok = j < a.n_cols;
+    double cj[kNumConsts];
+#pragma unroll
+    for (int q = 0; q < kNumConsts; ++q) cj[q] = sconst[(TILE + jl) * 12 + q];
+    const int il0 = wr * 32 + 4 * (lane >> 5);  // first row within the tile of this lane
+    const double Ld = (double)a.L;
+    const double invL = 1.0 / Ld;
+    const double tau2L = a.tau * a.tau * Ld;
+
+#define BS_STASH(BASE)                                                   \
+    _Pragma("unroll") for (int s_ = 0; s_ < 4; ++s_)                     \
+        _Pragma("unroll") for (int q_ = 0; q_ < 9; ++q_)                 \
+            sacc[(s_ * 9 + q_) * 64 + lane] = acc[q_][(BASE) + s_];
+
+#pragma unroll
+    for (int round = 0; round < 4; ++round) {
+        BS_STASH(4 * round)
+#pragma unroll 1
+        for (int sl = 0; sl < 4; sl += 2) {
+            // accumulator slot r = 4 round + sl holds row 8 (r/4) + r%4 of the
+            // lane's block; slots sl and sl + 1 are solved together
+            const int il = il0 + 8 * round + sl;
+            const int i0 = tr * TILE + il, i1 = i0 + 1;
+            double E0[9], E1[9], c0[kNumConsts], c1[kNumConsts];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                E0[q] = (double)sacc[(sl * 9 + q) * 64 + lane];
+                E1[q] = (double)sacc[((sl + 1) * 9 + q) * 64 + lane];
+            }
+#pragma unroll
+            for (int q = 0; q < kNumConsts; ++q) {
+                c0[q] = sconst[il * 12 + q];
+                c1[q] = sconst[(il + 1) * 12 + q];
+            }
+            ShiftedOut o[2];
+            qcp_shifted2(E0, c0, E1, c1, cj, o[0], o[1]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = h ? i1 : i0;
+                const double* ci = h ? c1 : c0;
+                const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
+                if (ok) {
+                    const double sgg = ci[10] * cj[10];
+                    const double err = a.acc_coef * sgg;
+                    // flag: unconverged / near-double root, near-duplicate
+                    // residuals, estimated error above tau (compared squared:
+                    // no FP64 sqrt), or residuals too large for the split
+                    const bool flag = !o[h].converged || (o[h].H < a.kappa_dup * a.u_rep * sgg) ||
+                                      (err * err > tau2L * o[h].rmsd2_L) ||
+                                      (2.0 * (ci[9] + cj[9]) > a.rep_g_max);
+                    const int64_t oi =
+                        a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
+                    if (a.out_f64)
+                        static_cast<double*>(a.out)[oi] = sqrt(o[h].rmsd2_L * invL);
+                    else
+                        static_cast<float*>(a.out)[oi] = sqrtf((float)(o[h].rmsd2_L * invL));
+                    flag_output(a, flag, oi);
+                }
+            }
+        }
+    }
+#undef BS_STASH
+}
+
+// One 64 x 64 tile of pairs by a 4-wave work-group: K loop over atom blocks
+// with the operand blocks double-buffered through LDS, then the epilogue.
+template <int MODE>
+__device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
+                                            uint4 (&lds)[2][4][kPanelBlockU4])
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+
+    const uint4* __restrict__ pa = static_cast<const uint4*>(a.panels_a);
+    const uint4* __restrict__ pb = static_cast<const uint4*>(a.panels_b);
+    const int64_t stride_cb = (int64_t)a.n_kblk * kPanelBlockU4;
+    const uint4* srcA0 = pa + (int64_t)(2 * tr) * stride_cb;
+    const uint4* srcB0 = pb + (int64_t)(2 * tc) * stride_cb;
+
+    // Six 16-byte loads per thread bring one stage (A0, A1, B0, B1 blocks) in.
+    auto src_of = [&](int u) -> const uint4* {
+        const int idx = tid + 256 * u;
+        const int blk = idx / kPanelBlockU4, off = idx - blk * kPanelBlockU4;
+        return (blk < 2 ? srcA0 + blk * stride_cb : srcB0 + (blk - 2) * stride_cb) + off;
+    };
+    const uint4* s0p = src_of(0);
+    const uint4* s1p = src_of(1);
+    const uint4* s2p = src_of(2);
+    const uint4* s3p = src_of(3);
+    const uint4* s4p = src_of(4);
+    const uint4* s5p = src_of(5);
+    uint4 st0, st1, st2, st3, st4, st5;
+#define BS_GLOAD(KB)                                          \
+    {                                                         \
+        const int64_t o_ = (int64_t)(KB) * kPanelBlockU4;     \
+        st0 = s0p[o_]; st1 = s1p[o_]; st2 = s2p[o_];          \
+        st3 = s3p[o_]; st4 = s4p[o_]; st5 = s5p[o_];          \
+    }
+#define BS_LSTORE(ST)                                         \
+    {                                                         \
+        uint4* d_ = &lds[(ST)][0][0] + tid;                   \
+        d_[0] = st0; d_[256] = st1; d_[512] = st2;            \
+        d_[768] = st3; d_[1024] = st4; d_[1280] = st5;        \
+    }
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
+
+    BS_GLOAD(0)
+    BS_LSTORE(0)
+    __syncthreads();
+    for (int kb = 0; kb < a.n_kblk; ++kb) {
+        const int cur = kb & 1;
+        const bool more = kb + 1 < a.n_kblk;
+        if (more) {
+            BS_GLOAD(kb + 1)
+        }
+        const uint4* A = lds[cur][wr];
+        const uint4* B = lds[cur][2 + wc];
+        mfma_block<MODE>(A, B, lane, acc);
+        if (more) {
+            BS_LSTORE(cur ^ 1)
+        }
+        __syncthreads();
+    }
+
+#undef BS_GLOAD
+#undef BS_LSTORE
+
+    epilogue<64>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), 256);
+}
+
+// One 32 x 32 tile of pairs by a single wave: operand fragments are read
+// straight from the panels (L2), no LDS staging and no barriers in the K loop,
+// then the epilogue.  Used for small ensembles, where 64 x 64 tiles would be
+// mostly padding.
+template <int MODE>
+__device__ __forceinline__ void gram_tile32(const GramArgs& a, int tr, int tc, char* lds)
+{
+    const int lane = threadIdx.x & 63;
+    const uint4* __restrict__ pa = static_cast<const uint4*>(a.panels_a);
+    const uint4* __restrict__ pb = static_cast<const uint4*>(a.panels_b);
+    const int64_t stride_cb = (int64_t)a.n_kblk * kPanelBlockU4;
+    const uint4* A = pa + (int64_t)tr * stride_cb;
+    const uint4* B = pb + (int64_t)tc * stride_cb;
+    f32x16 acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
+    for (int kb = 0; kb < a.n_kblk; ++kb)
+        mfma_block<MODE>(A + (int64_t)kb * kPanelBlockU4, B + (int64_t)kb * kPanelBlockU4, lane, acc);
+    epilogue<32>(a, tr, tc, acc, lds, 64);
+}
+
+}  // namespace bs

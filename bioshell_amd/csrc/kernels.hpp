@@ -69,11 +69,7 @@ struct GramArgs {
     int64_t out_ld;         // dense output: row stride (triangle: ignored)
     void* out;
     int out_f64;
-    const double* centred_a;  // FP64 centred rows (n_rows x L x 3): inline re-evaluation
-    const double* centred_b;  // FP64 centred columns
-    uint32_t* flag_pairs;   // (i, j) of pairs to re-evaluate in FP64
-    uint32_t* flag_count;
-    int max_flagged;
+    uint32_t* flag_bits;    // bit o set: output o is re-evaluated in FP64
     // flagging thresholds
     double kappa_dup;       // H < kappa_dup * u_rep * sqrt(g_i g_j)
     double u_rep;
@@ -94,10 +90,12 @@ void launch_gram(const GramArgs& a, GramMode mode, hipStream_t s);
 void launch_gram_pipe(const GramArgs& a, GramMode mode, hipStream_t s);
 void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
-// FP64 re-evaluation of flagged pairs from centred coordinates.
+// FP64 re-evaluation of the outputs whose bit is set in `bits` (n_out bits),
+// from centred coordinates; `count` (may be null) accumulates how many.
 void launch_fallback(const double* centred_a, const double* centred_b, int L,
-                     const uint32_t* pairs, const uint32_t* count, int max_pairs, int triangle,
-                     int n, int64_t out_ld, void* out, int out_f64, hipStream_t s);
+                     const uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld,
+                     void* out, int out_f64, uint32_t* count, hipStream_t s);
+void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s);
 
 // Per-query top-k over a dense n_q x n_lib score matrix (ascending, ties by index).
 void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, float* val,

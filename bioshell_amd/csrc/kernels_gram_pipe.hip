@@ -118,7 +118,6 @@ __global__ __launch_bounds__(256, 1) void gram_pipe_kernel(GramArgs a)
         const int pj = ptc * 64 + jl;
         const bool pjok = pj < a.n_cols;
         const double* pconst = sconst[pbuf];
-        uint32_t pflags = 0;
 
         // One K step: prefetch the next atom block (the last one is re-read at
         // the end, which keeps the step branch free), MFMAs, stage store, barrier.
@@ -167,7 +166,6 @@ __global__ __launch_bounds__(256, 1) void gram_pipe_kernel(GramArgs a)
         const double err = a.acc_coef * sgg;                                                 \
         const bool flag = ok & (!good | (sp_.H < dup_coef * sgg) | (err * err > tau2L * r2) | \
                                 (2.0 * (ci[9] + cjp[9]) > a.rep_g_max));                     \
-        pflags |= (flag ? 1u : 0u) << s_;                                                    \
         if (ok) {                                                                            \
             const int64_t oi =                                                               \
                 a.triangle ? tri_index(a.n_rows, i_, pj) : (int64_t)i_ * a.out_ld + pj;      \
@@ -175,6 +173,7 @@ __global__ __launch_bounds__(256, 1) void gram_pipe_kernel(GramArgs a)
                 static_cast<double*>(a.out)[oi] = sqrt(r2 * invL);                           \
             else                                                                             \
                 static_cast<float*>(a.out)[oi] = sqrtf((float)(r2 * invL));                  \
+            flag_output(a, flag, oi);                                                        \
         }                                                                                    \
     }
 
@@ -204,14 +203,6 @@ __global__ __launch_bounds__(256, 1) void gram_pipe_kernel(GramArgs a)
         for (; kb < nkc; ++kb) BS_KSTEP(kb)
 #undef BS_KSTEP
 #undef BS_ESLOT
-
-        // flagged pairs of the previous tile go to the FP64 re-evaluation list
-        if (have_prev && __ballot(pflags != 0u)) {
-            for (int s2 = 0; s2 < 16; ++s2) {
-                const int il = il0 + 8 * (s2 >> 2) + (s2 & 3);
-                append_flagged(a, ((pflags >> s2) & 1u) != 0u, ptr * 64 + il, pj, lane);
-            }
-        }
 
         // the current tile becomes the previous one
         if (have_cur) {
