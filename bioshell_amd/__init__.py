@@ -17,6 +17,8 @@ from .crmsd import (  # noqa: F401
     crmsd_all_vs_all,
     crmsd_all_vs_all_device,
     crmsd_all_vs_all_streamed,
+    crmsd_batch,
+    crmsd_batch_device,
     default_context,
     device_count,
     generate_ensemble,

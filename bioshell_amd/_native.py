@@ -57,6 +57,8 @@ SIGNATURES = {
     "bs_crmsd_all_vs_all_streamed": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_size, c_fp]),
     "bs_crmsd_batch": (c_status, [c_vp, ctypes.POINTER(c_dp), ctypes.POINTER(c_size), ctypes.POINTER(c_size),
                                   c_size, ctypes.c_int, ctypes.POINTER(c_fp)]),
+    "bs_crmsd_batch_device": (c_status, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_size),
+                                         ctypes.POINTER(c_size), c_size, ctypes.c_int, ctypes.POINTER(c_vp)]),
     "bs_autotune": (c_status, [c_vp, ctypes.c_int, ctypes.c_char_p]),
     "bs_autotune_load": (c_status, [c_vp, ctypes.c_char_p]),
     "bs_library_create": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.POINTER(c_vp)]),

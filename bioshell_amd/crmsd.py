@@ -127,6 +127,18 @@ class Context:
     def bytes_in_use(self) -> int:
         return _native.lib().bs_context_bytes_in_use(self.handle)
 
+    def autotune(self, method: "Method" = None, cache_path: str | None = None) -> None:
+        """Times 32x32 and 64x64 Gram tiles per (atom count, ensemble size) cell,
+        keeps the faster per cell for batches on this context, and writes the
+        table to ``cache_path`` (JSON) when given (bs_autotune)."""
+        m = Method.BF16X3_GRAM if method is None else method
+        path = cache_path.encode() if cache_path else None
+        check(_native.lib().bs_autotune(self.handle, int(m), path))
+
+    def autotune_load(self, cache_path: str) -> None:
+        """Loads a table written by :meth:`autotune` (bs_autotune_load)."""
+        check(_native.lib().bs_autotune_load(self.handle, cache_path.encode()))
+
 
 _default: Context | None = None
 
@@ -266,6 +278,38 @@ def crmsd_all_vs_all_streamed(models, method: Method = Method.BF16X3_GRAM, tile:
     check(_native.lib().bs_crmsd_all_vs_all_streamed(ctx.handle, _dp(x), n, L, int(method), tile,
                                                      out.ctypes.data_as(_native.c_fp)))
     return RmsdMatrix(n, out)
+
+
+def crmsd_batch(ensembles, method: Method = Method.BF16X3_GRAM,
+                ctx: Context | None = None) -> list[np.ndarray]:
+    """All-vs-all cRMSD of many independent ensembles (uneven n and L) in one
+    call (bs_crmsd_batch).  Returns each ensemble's packed f32 triangle."""
+    xs = [_ensemble(e) for e in ensembles]
+    ctx = ctx or default_context()
+    E = len(xs)
+    outs = [np.empty(max(x.shape[0] * (x.shape[0] - 1) // 2, 0), dtype=np.float32) for x in xs]
+    px = (_native.c_dp * max(E, 1))(*[_dp(x) for x in xs])
+    po = (_native.c_fp * max(E, 1))(*[o.ctypes.data_as(_native.c_fp) for o in outs])
+    nc = (ctypes.c_size_t * max(E, 1))(*[x.shape[0] for x in xs])
+    na = (ctypes.c_size_t * max(E, 1))(*[x.shape[1] for x in xs])
+    check(_native.lib().bs_crmsd_batch(ctx.handle, px, nc, na, E, int(method), po))
+    return outs
+
+
+def crmsd_batch_device(d_xyz, n_conf, n_atoms, d_out, method: Method = Method.BF16X3_GRAM,
+                       ctx: Context | None = None) -> None:
+    """Batch on device pointers: ``d_xyz[e]`` / ``d_out[e]`` are device addresses
+    (e.g. ``tensor.data_ptr()``).  Asynchronous on the context's stream
+    (bs_crmsd_batch_device)."""
+    ctx = ctx or default_context()
+    E = len(n_conf)
+    if not (len(d_xyz) == len(n_atoms) == len(d_out) == E):
+        raise ValueError("d_xyz, n_conf, n_atoms and d_out must have one entry per ensemble")
+    px = (ctypes.c_void_p * max(E, 1))(*[int(p) for p in d_xyz])
+    po = (ctypes.c_void_p * max(E, 1))(*[int(p) for p in d_out])
+    nc = (ctypes.c_size_t * max(E, 1))(*[int(v) for v in n_conf])
+    na = (ctypes.c_size_t * max(E, 1))(*[int(v) for v in n_atoms])
+    check(_native.lib().bs_crmsd_batch_device(ctx.handle, px, nc, na, E, int(method), po))
 
 
 # --------------------------------------------------------------- synthetic

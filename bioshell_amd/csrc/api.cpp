@@ -16,20 +16,6 @@ namespace {
 
 void check_ctx(const bs_context* ctx) { BS_REQUIRE(ctx != nullptr, "context is null"); }
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        BS_HIP(hipGetDevice(&prev));
-        if (prev != dev) BS_HIP(hipSetDevice(dev));
-    }
-    ~DeviceGuard()
-    {
-        int cur = -1;
-        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
 void check_shape(size_t n_conf, size_t n_atoms)
 {
     BS_REQUIRE(n_atoms >= 1, "n_atoms must be >= 1 (got %zu)", n_atoms);
@@ -81,6 +67,7 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         ctx->ws_xyz.bind(&ctx->budget);
         ctx->ws_out.bind(&ctx->budget);
         ctx->ws_misc.bind(&ctx->budget);
+        ctx->batch.bind(&ctx->budget);
         *out = ctx.release();
     });
 }
@@ -102,6 +89,7 @@ void bs_context_destroy(bs_context* ctx)
         ctx->ws_xyz.reset();
         ctx->ws_out.reset();
         ctx->ws_misc.reset();
+        ctx->batch.release();
         (void)hipStreamDestroy(ctx->stream);
     });
     delete ctx;

@@ -106,6 +106,23 @@ private:
     size_t bytes_ = 0;
 };
 
+// Makes `dev` the current device for a scope.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        BS_HIP(hipGetDevice(&prev));
+        if (prev != dev) BS_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+    DeviceGuard(const DeviceGuard&) = delete;
+    DeviceGuard& operator=(const DeviceGuard&) = delete;
+};
+
 // Event pair used to time a stretch of work on one stream.
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;

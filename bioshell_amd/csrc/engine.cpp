@@ -2,6 +2,7 @@
 // re-evaluation of flagged pairs, statistics.
 #include "engine.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -51,6 +52,31 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
     }
     launch_pack(centred, rot, ref, n, P.n_pad, L, P.n_kblk, mode, panels, consts, P.n_pad, s);
     BS_HIP(hipGetLastError());
+}
+
+GramThresholds gram_thresholds(GramMode mode, int L, int n_kblk)
+{
+    GramThresholds t;
+    // Error model of the residual Gram: bf16 hi/lo represents e to 2^-16
+    // relative (the dropped lo*lo term is bounded by 2^-16 sqrt(g_i g_j));
+    // FP32 accumulation adds ~2^-24 sqrt(#accumulations) sqrt(g_i g_j).
+    const double n_acc = mode == kGramBF16x3 ? 3.0 * n_kblk : 8.0 * n_kblk;
+    t.kappa_dup = 8.0;
+    t.u_rep = mode == kGramBF16x3 ? std::ldexp(1.0, -16) : std::ldexp(1.0, -24);
+    // Error of H - mu in units of sqrt(g_i g_j): FP32 accumulation
+    // (~2^-24 per accumulation, random-walk) plus, for the split, the
+    // dropped lo*lo products (|sum lo_i lo_j| <= 2^-16 sqrt(g_i g_j), a sum
+    // of 3L terms of random sign: 3 sigma = 3/sqrt(3L) of that bound).
+    t.acc_coef = 2.0 * std::ldexp(1.0, -24) * std::sqrt(n_acc);
+    if (mode == kGramBF16x3) t.acc_coef += t.u_rep * std::min(1.0, 3.0 / std::sqrt(3.0 * L));
+    t.tau = 2e-5;
+    // Rounding e to the split representation moves each coordinate by up
+    // to u_rep |e|; its effect on RMSD has a standard deviation of about
+    // u_rep sqrt(2 (g_i + g_j)) / (3 L).  Pairs whose 6-sigma value could
+    // exceed 5e-5 A (tiny L, large residuals) are re-evaluated in FP64.
+    const double r = 5e-5 * 3.0 * L / (6.0 * std::sqrt(2.0) * t.u_rep);
+    t.rep_g_max = r * r;
+    return t;
 }
 
 void RunRecord::create()
@@ -129,27 +155,12 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     g.out = run.d_out;
     g.out_f64 = run.out_f64 ? 1 : 0;
     g.flag_bits = flags;
-    // Error model of the residual Gram: bf16 hi/lo represents e to 2^-16
-    // relative (the dropped lo*lo term is bounded by 2^-16 sqrt(g_i g_j));
-    // FP32 accumulation adds ~2^-24 sqrt(#accumulations) sqrt(g_i g_j).
-    const double n_acc = A.mode == kGramBF16x3 ? 3.0 * A.n_kblk : 8.0 * A.n_kblk;
-    g.kappa_dup = 8.0;
-    g.u_rep = A.mode == kGramBF16x3 ? std::ldexp(1.0, -16) : std::ldexp(1.0, -24);
-    // Error of H - mu in units of sqrt(g_i g_j): FP32 accumulation
-    // (~2^-24 per accumulation, random-walk) plus, for the split, the
-    // dropped lo*lo products (|sum lo_i lo_j| <= 2^-16 sqrt(g_i g_j), a sum
-    // of 3L terms of random sign: 3 sigma = 3/sqrt(3L) of that bound).
-    g.acc_coef = 2.0 * std::ldexp(1.0, -24) * std::sqrt(n_acc);
-    if (A.mode == kGramBF16x3) g.acc_coef += g.u_rep * std::min(1.0, 3.0 / std::sqrt(3.0 * A.L));
-    g.tau = 2e-5;
-    // Rounding e to the split representation moves each coordinate by up
-    // to u_rep |e|; its effect on RMSD has a standard deviation of about
-    // u_rep sqrt(2 (g_i + g_j)) / (3 L).  Pairs whose 6-sigma value could
-    // exceed 5e-5 A (tiny L, large residuals) are re-evaluated in FP64.
-    {
-        const double t = 5e-5 * 3.0 * A.L / (6.0 * std::sqrt(2.0) * g.u_rep);
-        g.rep_g_max = t * t;
-    }
+    const GramThresholds th = gram_thresholds(A.mode, A.L, A.n_kblk);
+    g.kappa_dup = th.kappa_dup;
+    g.u_rep = th.u_rep;
+    g.acc_coef = th.acc_coef;
+    g.tau = th.tau;
+    g.rep_g_max = th.rep_g_max;
     g.max_wg = sc.max_wg > 0 ? sc.max_wg : 512;
     g.skew_sleeps = sc.skew_sleeps;
 

@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "batch.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 #include "qcp.hpp"
@@ -40,6 +41,12 @@ size_t prepared_bytes(int n, int L);
 // one Procrustes-mean step from conformation 0.
 void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
              const double* fixed_ref, hipStream_t s);
+
+// Flagging thresholds of the Gram epilogue's error model at atom count L.
+struct GramThresholds {
+    double kappa_dup, u_rep, acc_coef, tau, rep_g_max;
+};
+GramThresholds gram_thresholds(GramMode mode, int L, int n_kblk);
 
 struct GramRun {
     bool triangle = true;
@@ -114,7 +121,8 @@ struct bs_context {
     size_t ring_next = 0, ring_used = 0;
     uint32_t* h_counts = nullptr;      // pinned, one per record
     bs::RunRecord* begin_record();     // next ring slot, recycled
-    std::vector<int> tune_variant;  // autotuner: Gram tile variant per atom-count bucket
+    std::vector<bs::TuneCell> tune[4];  // autotuner: Gram tile side per (L, n) cell, per method
+    bs::BatchWorkspace batch;          // batched pipeline (bs_crmsd_batch*)
     int n_cu = 0;
     std::mutex mu;
 };

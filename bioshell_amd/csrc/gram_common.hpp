@@ -3,6 +3,7 @@
 
 #include "kernels.hpp"
 #include "qcp.hpp"
+#include "wave.hpp"
 
 namespace bs {
 
@@ -75,6 +76,40 @@ __device__ __forceinline__ void mfma_block(const uint4* __restrict__ A, const ui
                     acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[c][s], fb[d][s],
                                                                           acc[3 * c + d], 0, 0, 0);
     }
+}
+
+// Row of packed-triangle position o (row-major upper triangle of n, i < j).
+__device__ inline int tri_row(int64_t o, int n)
+{
+    const double b = 2.0 * n - 1.0;
+    int i = (int)floor((b - sqrt(b * b - 8.0 * (double)o)) * 0.5);
+    auto start = [n](int64_t r) { return r * n - r * (r + 1) / 2; };
+    while (i > 0 && start(i) > o) --i;
+    while (start(i + 1) <= o) ++i;
+    return i;
+}
+
+// FP64 cRMSD of one pair from centred coordinates x, y (L atoms each), by one
+// wave: the 64 lanes split the atoms, then the textbook QCP with the
+// near-degenerate polish.  Every lane returns the result.
+__device__ inline double fallback_pair(const double* __restrict__ x, const double* __restrict__ y,
+                                       int L, int lane)
+{
+    double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ga = 0.0, gb = 0.0;
+    for (int k = lane; k < L; k += 64) {
+        const double a0 = x[3 * k], a1 = x[3 * k + 1], a2 = x[3 * k + 2];
+        const double b0 = y[3 * k], b1 = y[3 * k + 1], b2 = y[3 * k + 2];
+        m9[0] += a0 * b0; m9[1] += a0 * b1; m9[2] += a0 * b2;
+        m9[3] += a1 * b0; m9[4] += a1 * b1; m9[5] += a1 * b2;
+        m9[6] += a2 * b0; m9[7] += a2 * b1; m9[8] += a2 * b2;
+        ga += a0 * a0 + a1 * a1 + a2 * a2;
+        gb += b0 * b0 + b1 * b1 + b2 * b2;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m9[q] = wave_sum(m9[q]);
+    ga = wave_sum(ga);
+    gb = wave_sum(gb);
+    return qcp_rmsd<double>(m9, ga, gb, L);
 }
 
 // Marks output o for FP64 re-evaluation.

@@ -5,8 +5,16 @@
 
 namespace bs {
 
+// LDS bytes the epilogue of a TILE x TILE tile needs (see below).
+template <int TILE>
+constexpr int epilogue_lds_bytes()
+{
+    return 2 * TILE * 12 * 8 + (TILE / 32) * (TILE / 32) * 4 * 9 * 64 * 4;
+}
+
 // The FP64 QCP epilogue of a TILE x TILE tile whose accumulators the calling
-// waves hold (wave w owns rows 32 (w/2) .. +31 and columns 32 (w%2) .. +31).
+// waves hold: TILE 64 is four waves of a work-group (wave w owns rows
+// 32 (w/2) .. +31 and columns 32 (w%2) .. +31), TILE 32 is one wave on its own.
 // LDS layout (bytes) at `lds`, which the caller no longer needs:
 //   [0, 2 TILE 96)        constants of the tile's rows and columns, 12 doubles each
 //   then, per wave        4 accumulator slots x 9 values x 64 lanes (f32)
@@ -14,14 +22,19 @@ namespace bs {
 // FP64 solve serves all 16 pairs a lane owns.
 template <int TILE>
 __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x16 (&acc)[9],
-                                         char* lds, int nthreads)
+                                         char* lds, int thread)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int kWaves = (TILE / 32) * (TILE / 32);
+    constexpr int kThreads = 64 * kWaves;
+    const int lane = thread & 63;
+    const int tid = kWaves == 1 ? lane : thread;
+    const int wv = kWaves == 1 ? 0 : (thread >> 6);
     const int wr = wv >> 1, wc = wv & 1;
     constexpr int kConstBytes = 2 * TILE * 12 * 8;
     double* sconst = reinterpret_cast<double*>(lds);
     float* sacc = reinterpret_cast<float*>(lds + kConstBytes) + wv * (4 * 9 * 64);
-    for (int v = tid; v < kNumConsts * 2 * TILE; v += nthreads) {
+    if constexpr (kWaves == 1) wave_lds_sync();  // earlier reads of this region are done
+    for (int v = tid; v < kNumConsts * 2 * TILE; v += kThreads) {
         const int q = v / (2 * TILE), r = v % (2 * TILE);
         const bool is_row = r < TILE;
         const int conf = is_row ? tr * TILE + r : tc * TILE + (r - TILE);
@@ -29,7 +42,10 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
                                   : (conf < a.n_cols ? a.consts_b[(int64_t)q * a.ldb + conf] : 0.0);
         sconst[r * 12 + q] = val;
     }
-    __syncthreads();
+    if constexpr (kWaves == 1)
+        wave_lds_sync();
+    else
+        __syncthreads();
 
     const int jl = wc * 32 + (lane & 31);  // column within the tile
     const int j = tc * TILE + jl;
@@ -49,7 +65,7 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
 
 #pragma unroll
     for (int round = 0; round < 4; ++round) {
-        BS_STASH(4 * round)
+        BS_STASH(4 * round)  // each lane reads back only its own slots: no sync
 #pragma unroll 1
         for (int sl = 0; sl < 4; sl += 2) {
             // accumulator slot r = 4 round + sl holds row 8 (r/4) + r%4 of the
@@ -101,9 +117,9 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
 // with the operand blocks double-buffered through LDS, then the epilogue.
 template <int MODE>
 __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
-                                            uint4 (&lds)[2][4][kPanelBlockU4])
+                                            uint4 (&lds)[2][4][kPanelBlockU4], int thread)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = thread, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
 
     const uint4* __restrict__ pa = static_cast<const uint4*>(a.panels_a);
@@ -165,17 +181,18 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
 #undef BS_GLOAD
 #undef BS_LSTORE
 
-    epilogue<64>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), 256);
+    epilogue<64>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
 }
 
 // One 32 x 32 tile of pairs by a single wave: operand fragments are read
 // straight from the panels (L2), no LDS staging and no barriers in the K loop,
-// then the epilogue.  Used for small ensembles, where 64 x 64 tiles would be
+// then the epilogue in this wave's own LDS region (epilogue_lds_bytes<32>()
+// bytes at `lds`).  Used for small ensembles, where 64 x 64 tiles would be
 // mostly padding.
 template <int MODE>
-__device__ __forceinline__ void gram_tile32(const GramArgs& a, int tr, int tc, char* lds)
+__device__ __forceinline__ void gram_tile32(const GramArgs& a, int tr, int tc, char* lds, int thread)
 {
-    const int lane = threadIdx.x & 63;
+    const int lane = thread & 63;
     const uint4* __restrict__ pa = static_cast<const uint4*>(a.panels_a);
     const uint4* __restrict__ pb = static_cast<const uint4*>(a.panels_b);
     const int64_t stride_cb = (int64_t)a.n_kblk * kPanelBlockU4;
@@ -188,7 +205,7 @@ __device__ __forceinline__ void gram_tile32(const GramArgs& a, int tr, int tc, c
         for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
     for (int kb = 0; kb < a.n_kblk; ++kb)
         mfma_block<MODE>(A + (int64_t)kb * kPanelBlockU4, B + (int64_t)kb * kPanelBlockU4, lane, acc);
-    epilogue<32>(a, tr, tc, acc, lds, 64);
+    epilogue<32>(a, tr, tc, acc, lds, thread);
 }
 
 }  // namespace bs

@@ -101,6 +101,59 @@ void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s);
 void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, float* val,
                  hipStream_t s);
 
+// ------------------------------------------------------------ batches
+// One ensemble of a batch, as the batched kernels see it (device memory).
+// Per-conformation arrays (centred, rot) are slices of batch-wide arrays.
+struct BatchEns {
+    const double* xyz;      // n x L x 3 f64 input
+    double* centred;        // n x L x 3
+    double* rot;            // n x 9 (covariance, then rotation)
+    double* ref;            // 3L + 6
+    double* partial;        // n_slices x 3L
+    void* panels;           // n_pad / 32 x n_kblk blocks
+    double* consts;         // kNumConsts x n_pad
+    uint32_t* bits;         // flag bitmap of the n(n-1)/2 outputs
+    float* out;             // packed triangle, f32
+    int n, L, n_pad, n_kblk;
+    int n_slices, tile;     // tile: 32 or 64
+    double acc_coef, rep_g_max;  // error-model thresholds at this L
+};
+
+// A batch: ensemble table, the prefix sums that map a batch-wide index
+// (conformation, padded conformation, mean slice, reference coordinate,
+// bitmap word) to its ensemble, and the Gram work lists.
+//
+// Work items are dealt to 8 per-XCD lists (whole ensembles per list, costs
+// balanced); a work-group on XCD x pulls items from list x through an atomic
+// counter and then helps the other lists.  An item is either one 64 x 64
+// tile (bit 31 clear: index into tiles64) or four 32 x 32 tiles, one per
+// wave (bit 31 set: index q, tiles32[4q .. 4q+3]).  Tile codes are
+// (ensemble << 32) | (tile row << 16) | tile col; ~0 marks an empty slot.
+struct BatchArgs {
+    const BatchEns* ens;
+    int n_ens;
+    int mode;
+    const int* conf_prefix;
+    const int* pad_prefix;
+    const int* slice_prefix;
+    const int* coord_prefix;
+    const int64_t* word_prefix;
+    int total_conf, total_pad, total_slices, total_coords;
+    int64_t total_words;
+    const uint32_t* items;
+    const int* list_start;  // 9 entries: list x is items[list_start[x] .. list_start[x+1])
+    const uint64_t* tiles64;
+    const uint64_t* tiles32;
+    uint32_t* counters;     // 8 list counters, then the flagged-pair count
+    double kappa_dup, u_rep, tau;
+    int n_wg;               // persistent work-groups of the Gram kernel
+};
+
+// The batched pipeline's launches, in order (each over all ensembles).
+void launch_batch_prep(const BatchArgs& b, double* rot_all, hipStream_t s);
+void launch_batch_gram(const BatchArgs& b, uint32_t* zero_base, int64_t zero_words, hipStream_t s);
+void launch_batch_fallback(const BatchArgs& b, hipStream_t s);
+
 // Single-pair superposition (rotation + translation) in FP64.
 void launch_superpose(const double* a, const double* b, int L, double* out13, hipStream_t s);
 

@@ -1,0 +1,284 @@
+// kernels_batch.hip -- many ensembles of uneven size (n conformations, L
+// atoms each) in a fixed sequence of launches, independent of how many
+// ensembles there are: every preparation step is one launch over all the
+// batch's conformations (or slices, coordinates, ...), the Gram step is one
+// persistent launch over a work list of (ensemble, tile) items, and the FP64
+// re-evaluation is one launch over all flag bitmaps.  The host captures the
+// sequence in a HIP graph.
+//
+// Batch-wide index -> (ensemble, local index) goes through the prefix sums of
+// BatchArgs by binary search (ensembles are a few hundred; the search is a
+// handful of scalar loads per wave).
+#include "gram_tile.hpp"
+#include "prep_device.hpp"
+
+#include <algorithm>
+
+namespace bs {
+namespace {
+
+// Ensemble e with prefix[e] <= g < prefix[e + 1] (empty ranges are skipped).
+template <typename T>
+__device__ inline int find_ens(const T* __restrict__ prefix, int n_ens, T g)
+{
+    int lo = 0, hi = n_ens;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------ prep
+__global__ __launch_bounds__(256) void batch_centre_cov0_kernel(BatchArgs b)
+{
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= b.total_conf) return;
+    const int e = find_ens(b.conf_prefix, b.n_ens, g);
+    const BatchEns& E = b.ens[e];
+    const int i = g - b.conf_prefix[e];
+    centre_one<true>(E.xyz + (int64_t)i * E.L * 3, E.L, E.centred + (int64_t)i * E.L * 3, E.xyz,
+                     E.rot + (int64_t)i * 9, threadIdx.x & 63);
+}
+
+// rot_all is the batch-wide rotation array (ensemble e's slice starts at
+// conformation conf_prefix[e]), so no lookup is needed.
+__global__ __launch_bounds__(64) void batch_rot_kernel(double* __restrict__ rot_all, int total)
+{
+    const int g = blockIdx.x * 64 + threadIdx.x;
+    if (g >= total) return;
+    rot_one(rot_all + (int64_t)g * 9);
+}
+
+__global__ __launch_bounds__(256) void batch_mean_partial_kernel(BatchArgs b)
+{
+    __shared__ double sR[kMeanSliceConfs][9];
+    const int g = blockIdx.x;
+    const int e = find_ens(b.slice_prefix, b.n_ens, g);
+    const BatchEns& E = b.ens[e];
+    mean_partial_block(E.centred, E.rot, E.n, E.L, g - b.slice_prefix[e], E.partial, sR);
+}
+
+__global__ __launch_bounds__(256) void batch_mean_final_kernel(BatchArgs b)
+{
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= b.total_coords) return;
+    const int e = find_ens(b.coord_prefix, b.n_ens, g);
+    const BatchEns& E = b.ens[e];
+    mean_final_wave(E.partial, E.n_slices, E.n, E.L, g - b.coord_prefix[e], E.ref,
+                    threadIdx.x & 63);
+}
+
+__global__ __launch_bounds__(256) void batch_ref_gram_kernel(BatchArgs b)
+{
+    __shared__ double part[4][6];
+    const BatchEns& E = b.ens[blockIdx.x];
+    ref_gram_block(E.ref, E.L, part);
+}
+
+__global__ __launch_bounds__(256) void batch_fit_cov_kernel(BatchArgs b)
+{
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= b.total_conf) return;
+    const int e = find_ens(b.conf_prefix, b.n_ens, g);
+    const BatchEns& E = b.ens[e];
+    const int i = g - b.conf_prefix[e];
+    fit_cov_one(E.centred + (int64_t)i * E.L * 3, E.L, E.ref, E.rot + (int64_t)i * 9,
+                threadIdx.x & 63);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void batch_pack_kernel(BatchArgs b)
+{
+    __shared__ uint4 img[4][kPackWordsU4];
+    const int wv = threadIdx.x >> 6;
+    const int g = blockIdx.x * 4 + wv;
+    if (g >= b.total_pad) return;
+    const int e = find_ens(b.pad_prefix, b.n_ens, g);
+    const BatchEns& E = b.ens[e];
+    pack_one<MODE>(E.centred, E.rot, E.ref, E.n, E.L, E.n_kblk, g - b.pad_prefix[e],
+                   static_cast<uint4*>(E.panels), E.consts, E.n_pad, img[wv], threadIdx.x & 63);
+}
+
+__global__ void batch_zero_kernel(uint4* p, int64_t n4)
+{
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n4; v += (int64_t)gridDim.x * 256)
+        p[v] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// ------------------------------------------------------------------ Gram
+__device__ inline uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The view of ensemble E that the tile code takes.
+__device__ inline GramArgs ens_args(const BatchArgs& b, const BatchEns& E)
+{
+    GramArgs a{};
+    a.panels_a = E.panels;
+    a.panels_b = E.panels;
+    a.consts_a = E.consts;
+    a.consts_b = E.consts;
+    a.lda = E.n_pad;
+    a.ldb = E.n_pad;
+    a.n_rows = E.n;
+    a.n_cols = E.n;
+    a.L = E.L;
+    a.n_kblk = E.n_kblk;
+    a.triangle = 1;
+    a.out_ld = 0;
+    a.out = E.out;
+    a.out_f64 = 0;
+    a.flag_bits = E.bits;
+    a.kappa_dup = b.kappa_dup;
+    a.u_rep = b.u_rep;
+    a.acc_coef = E.acc_coef;
+    a.tau = b.tau;
+    a.rep_g_max = E.rep_g_max;
+    return a;
+}
+
+constexpr int kBatchLdsBytes = 4 * epilogue_lds_bytes<32>();  // 60 KiB: four 32-tile waves
+static_assert(kBatchLdsBytes >= 2 * 4 * kPanelBlockU4 * 16, "64-tile staging must fit");
+static_assert(kBatchLdsBytes >= epilogue_lds_bytes<64>(), "64-tile epilogue must fit");
+
+// Persistent: n_wg work-groups (2 per CU), each pulls items from its XCD's
+// list, then from the others in turn, until every list is drained.  Every
+// work-group leaves when its last probe of list x + 7 comes back empty.
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void batch_gram_kernel(BatchArgs b)
+{
+    __shared__ __align__(16) char lds[kBatchLdsBytes];
+    __shared__ uint32_t s_item;
+    const int xcd = blockIdx.x % kXcd;
+    int probe = 0;  // lists xcd, xcd + 1, ... (mod 8) in turn
+    while (true) {
+        if (threadIdx.x == 0) {
+            uint32_t item = 0xFFFFFFFFu;
+            while (probe < kXcd) {
+                const int x = (xcd + probe) % kXcd;
+                const int len = b.list_start[x + 1] - b.list_start[x];
+                if (len > 0) {
+                    const uint32_t k = atomicAdd(b.counters + x, 1u);
+                    if (k < (uint32_t)len) {
+                        item = b.items[b.list_start[x] + (int)k];
+                        break;
+                    }
+                }
+                ++probe;
+            }
+            s_item = item;
+        }
+        __syncthreads();
+        // uniform values in scalar registers: the ensemble's fields are then
+        // scalar loads, not per-lane copies
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();  // all have read s_item before thread 0 writes the next one
+        if (item == 0xFFFFFFFFu) break;
+        // an opaque copy of the thread index per item, so the compiler cannot
+        // hoist thread-derived addresses out of the loop and keep them live
+        // across items (that costs ~50 VGPRs and spills)
+        int thread = threadIdx.x;
+        asm volatile("" : "+v"(thread));
+        if (!(item & 0x80000000u)) {
+            const uint64_t code = uniform64(b.tiles64[item]);
+            const BatchEns& E = b.ens[(int)(code >> 32)];
+            const GramArgs a = ens_args(b, E);
+            gram_tile64<MODE>(a, (int)((code >> 16) & 0xFFFFu), (int)(code & 0xFFFFu),
+                              *reinterpret_cast<uint4(*)[2][4][kPanelBlockU4]>(lds), thread);
+            __syncthreads();  // the next item reuses the LDS
+        } else {
+            const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            const uint64_t code = uniform64(b.tiles32[4 * (int64_t)(item & 0x7FFFFFFFu) + w]);
+            if (code != ~0ull) {
+                const BatchEns& E = b.ens[(int)(code >> 32)];
+                const GramArgs a = ens_args(b, E);
+                gram_tile32<MODE>(a, (int)((code >> 16) & 0xFFFFu), (int)(code & 0xFFFFu),
+                                  lds + w * epilogue_lds_bytes<32>(), thread);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ FP64 re-evaluation
+// Each wave takes 64 consecutive words of the batch-wide bitmap (one per
+// lane) and re-evaluates their set bits one pair at a time.
+__global__ __launch_bounds__(256) void batch_fallback_kernel(BatchArgs b)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t n_waves = (int64_t)gridDim.x * 4;
+    const uint32_t* bits0 = b.ens[0].bits;  // the bitmaps are one array, ensemble 0 first
+    for (int64_t w0 = wave * 64; w0 < b.total_words; w0 += n_waves * 64) {
+        const int64_t wi = w0 + lane;
+        uint32_t word = wi < b.total_words ? bits0[wi] : 0u;
+        uint32_t done = 0;
+        while (true) {
+            const uint64_t m = __ballot(word != 0u);
+            if (!m) break;
+            const int src = __ffsll((unsigned long long)m) - 1;
+            const uint32_t wsrc = __shfl(word, src);
+            const int bit = __ffs(wsrc) - 1;
+            if (lane == src) word &= word - 1u;  // clear the lowest set bit
+            const int64_t gw = w0 + src;
+            const int e = find_ens(b.word_prefix, b.n_ens, gw);
+            const BatchEns& E = b.ens[e];
+            const int64_t o = (gw - b.word_prefix[e]) * 32 + bit;
+            const int64_t n_out = (int64_t)E.n * (E.n - 1) / 2;
+            if (o >= n_out) continue;
+            const int i = tri_row(o, E.n);
+            const int j = (int)(o - ((int64_t)i * E.n - (int64_t)i * (i + 1) / 2)) + i + 1;
+            const double r = fallback_pair(E.centred + (int64_t)i * E.L * 3,
+                                           E.centred + (int64_t)j * E.L * 3, E.L, lane);
+            if (lane == 0) E.out[o] = (float)r;
+            ++done;
+        }
+        if (lane == 0 && done) atomicAdd(b.counters + kXcd, done);
+    }
+}
+
+}  // namespace
+
+void launch_batch_prep(const BatchArgs& b, double* rot_all, hipStream_t s)
+{
+    const int cw = (b.total_conf + 3) / 4;
+    batch_centre_cov0_kernel<<<cw, 256, 0, s>>>(b);
+    batch_rot_kernel<<<(b.total_conf + 63) / 64, 64, 0, s>>>(rot_all, b.total_conf);
+    batch_mean_partial_kernel<<<b.total_slices, 256, 0, s>>>(b);
+    batch_mean_final_kernel<<<(b.total_coords + 3) / 4, 256, 0, s>>>(b);
+    batch_ref_gram_kernel<<<b.n_ens, 256, 0, s>>>(b);
+    batch_fit_cov_kernel<<<cw, 256, 0, s>>>(b);
+    batch_rot_kernel<<<(b.total_conf + 63) / 64, 64, 0, s>>>(rot_all, b.total_conf);
+    const int pw = (b.total_pad + 3) / 4;
+    if (b.mode == kGramBF16x3)
+        batch_pack_kernel<kGramBF16x3><<<pw, 256, 0, s>>>(b);
+    else
+        batch_pack_kernel<kGramFP32><<<pw, 256, 0, s>>>(b);
+}
+
+void launch_batch_gram(const BatchArgs& b, uint32_t* zero_base, int64_t zero_words, hipStream_t s)
+{
+    // bitmaps and list counters are one contiguous, 16-byte-aligned region
+    const int64_t n4 = (zero_words + 3) / 4;
+    const int64_t zb = std::min<int64_t>(2048, std::max<int64_t>(1, (n4 + 255) / 256));
+    batch_zero_kernel<<<(int)zb, 256, 0, s>>>(reinterpret_cast<uint4*>(zero_base), n4);
+    if (b.mode == kGramBF16x3)
+        batch_gram_kernel<kGramBF16x3><<<b.n_wg, 256, 0, s>>>(b);
+    else
+        batch_gram_kernel<kGramFP32><<<b.n_wg, 256, 0, s>>>(b);
+}
+
+void launch_batch_fallback(const BatchArgs& b, hipStream_t s)
+{
+    const int64_t waves = (b.total_words + 63) / 64;
+    const int64_t blocks = std::min<int64_t>(2048, std::max<int64_t>(1, (waves + 3) / 4));
+    batch_fallback_kernel<<<(int)blocks, 256, 0, s>>>(b);
+}
+
+}  // namespace bs

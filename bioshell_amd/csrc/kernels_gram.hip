@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
     if (a.skew_sleeps > 0 && wg >= a.max_wg / 2 && wg < a.max_wg)
         for (int k = 0; k < a.skew_sleeps; ++k) __builtin_amdgcn_s_sleep(127);
     const uint32_t code = a.tiles[t];
-    gram_tile64<MODE>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds);
+    gram_tile64<MODE>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds, (int)threadIdx.x);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n)
@@ -55,24 +55,6 @@ __global__ void zero_words_kernel(uint4* p, int64_t n4)
 {
     for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n4; v += (int64_t)gridDim.x * 256)
         p[v] = make_uint4(0u, 0u, 0u, 0u);
-}
-
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-// Row of packed-triangle position o (row-major upper triangle of n, i < j).
-__device__ inline int tri_row(int64_t o, int n)
-{
-    const double b = 2.0 * n - 1.0;
-    int i = (int)floor((b - sqrt(b * b - 8.0 * (double)o)) * 0.5);
-    auto start = [n](int64_t r) { return r * n - r * (r + 1) / 2; };
-    while (i > 0 && start(i) > o) --i;
-    while (start(i + 1) <= o) ++i;
-    return i;
 }
 
 // FP64 re-evaluation of flagged outputs: each wave takes 64 bitmap words (one
@@ -110,24 +92,8 @@ __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict_
                 i = (int)(o / out_ld);
                 j = (int)(o - (int64_t)i * out_ld);
             }
-            const double* x = ca + (int64_t)i * L * 3;
-            const double* y = cbp + (int64_t)j * L * 3;
-            double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ga = 0.0, gb = 0.0;
-            for (int k = lane; k < L; k += 64) {
-                const double a0 = x[3 * k], a1 = x[3 * k + 1], a2 = x[3 * k + 2];
-                const double b0 = y[3 * k], b1 = y[3 * k + 1], b2 = y[3 * k + 2];
-                m9[0] += a0 * b0; m9[1] += a0 * b1; m9[2] += a0 * b2;
-                m9[3] += a1 * b0; m9[4] += a1 * b1; m9[5] += a1 * b2;
-                m9[6] += a2 * b0; m9[7] += a2 * b1; m9[8] += a2 * b2;
-                ga += a0 * a0 + a1 * a1 + a2 * a2;
-                gb += b0 * b0 + b1 * b1 + b2 * b2;
-            }
-#pragma unroll
-            for (int q = 0; q < 9; ++q) m9[q] = wave_sum(m9[q]);
-            ga = wave_sum(ga);
-            gb = wave_sum(gb);
+            const double r = fallback_pair(ca + (int64_t)i * L * 3, cbp + (int64_t)j * L * 3, L, lane);
             if (lane == 0) {
-                const double r = qcp_rmsd<double>(m9, ga, gb, L);
                 if (out_f64)
                     static_cast<double*>(out)[o] = r;
                 else

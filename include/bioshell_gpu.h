@@ -157,8 +157,23 @@ bs_status bs_crmsd_batch(bs_context* ctx, const double* const* xyz, const size_t
                          const size_t* n_atoms, size_t n_ens, bs_method method,
                          float* const* out_upper);
 
-/* Runs the tile-shape autotuner for `method` over representative shapes and
- * stores its choices in `cache_path` (JSON); loads them into the context. */
+/* The same on device-resident buffers: d_xyz[e] (n_conf[e] x n_atoms[e] x 3
+ * f64) and d_out_upper[e] (n_conf[e](n_conf[e]-1)/2 f32) are device
+ * pointers.  Asynchronous on the context's stream; the pointer arrays
+ * themselves are read before the call returns.  Ensembles with fewer than 2
+ * conformations are skipped.  The launch sequence (all preparation steps, one
+ * persistent Gram launch over every ensemble's tiles, one FP64 re-evaluation
+ * launch) is a HIP graph, built on the first call of a batch shape and
+ * replayed after that. */
+bs_status bs_crmsd_batch_device(bs_context* ctx, const double* const* d_xyz,
+                                const size_t* n_conf, const size_t* n_atoms, size_t n_ens,
+                                bs_method method, float* const* d_out_upper);
+
+/* Runs the tile-shape autotuner for `method` (a Gram method) over
+ * representative batches -- atom counts up to 64, 128, 256, 512 and beyond,
+ * times conformation counts up to 64, 128, 256, 512 and beyond -- timing
+ * 32 x 32 and 64 x 64 tiles for each cell, stores the faster shape per cell
+ * in `cache_path` (JSON; may be null) and loads the table into the context. */
 bs_status bs_autotune(bs_context* ctx, bs_method method, const char* cache_path);
 /* Loads a previously written autotune cache into the context. */
 bs_status bs_autotune_load(bs_context* ctx, const char* cache_path);
