@@ -9,7 +9,11 @@ flagged pairs, writing the 49,995,000-value packed f32 triangle to HBM.
 
 Also measured in the same run: the plain per-pair GPU kernel (FP32) on the
 same input, the error of both against the FP64 oracle on a row sample, and the
-oracle itself on the host cores (cpu_baseline, a bounded row sample).
+oracle itself on the host cores (cpu_baseline, a bounded row sample).  Two
+more legs report the other north-star workloads beside the headline line:
+"batch" (512 ensembles of uneven n and L as one graph launch, autotuned tiles)
+and "serving" (query-vs-library top-100 over device-resident libraries,
+Poisson arrivals at a fixed rate, p50/p99 latency and pairs/s).
 
 Prints one JSON line.
 """
@@ -49,6 +53,145 @@ def traffic_bytes(n: int, L: int) -> int | None:
     return None
 
 
+def batch_leg(args, B, torch, log) -> dict:
+    """512 independent ensembles, n ~ U(20, 500), L ~ U(30, 800), sigma ~ U(0.5, 3):
+    autotuned tiles, the whole batch as one graph launch, inputs resident in HBM."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(3)
+    E = args.batch_ensembles
+    ns = rng.integers(20, 501, E)
+    Ls = rng.integers(30, 801, E)
+    log(f"batch: generating {E} ensembles ({int((ns * Ls).sum() * 24) >> 20} MiB)")
+    xs = [B.generate_ensemble(7000 + e, int(ns[e]), int(Ls[e]), sigma=(0.5, 3.0)) for e in range(E)]
+    pairs = int(sum(n * (n - 1) // 2 for n in ns))
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(x).to(dev) for x in xs]
+    d_out = [torch.empty(int(n) * (int(n) - 1) // 2, dtype=torch.float32, device=dev) for n in ns]
+    torch.cuda.synchronize()
+    ctx = B.Context(0)
+    t0 = time.perf_counter()
+    ctx.autotune(B.Method.BF16X3_GRAM, None)
+    tune_s = time.perf_counter() - t0
+    ptr_in = [t.data_ptr() for t in d_in]
+    ptr_out = [t.data_ptr() for t in d_out]
+
+    def step(method):
+        B.crmsd_batch_device(ptr_in, ns, Ls, ptr_out, method, ctx=ctx)
+
+    t0 = time.perf_counter()
+    step(B.Method.BF16X3_GRAM)  # builds the graph
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(args.warmup):
+        step(B.Method.BF16X3_GRAM)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(B.Method.BF16X3_GRAM)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    dev_ms = float(np.median([h["total_ms"] for h in ctx.history(args.steps)]))
+    flagged = int(ctx.last_stats()["flagged_pairs"])
+    log(f"batch bf16x3: {dt * 1e3:.3f} ms/step (device {dev_ms:.3f} ms), {pairs} pairs")
+    got = [t.cpu().numpy() for t in d_out]
+    # error against the oracle on the 8 ensembles with the fewest pairs x atoms
+    order = np.argsort(ns * ns * Ls)[:8]
+    err = max(float(np.abs(got[e].astype(np.float64) - O.all_vs_all(xs[e])).max()) for e in order)
+    # plain per-pair kernel over the same batch (one launch per ensemble)
+    step(B.Method.FP32_PLAIN)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        step(B.Method.FP32_PLAIN)
+    torch.cuda.synchronize()
+    plain_dt = (time.perf_counter() - t0) / 2
+    # host buffers in and out (PCIe and staging included)
+    t0 = time.perf_counter()
+    B.crmsd_batch(xs, B.Method.BF16X3_GRAM, ctx=ctx)
+    host_dt = time.perf_counter() - t0
+    ctx.close()
+    return {
+        "workload": f"{E} ensembles, n ~ U(20,500), L ~ U(30,800), sigma U(0.5,3) A; "
+                    "BF16X3_GRAM, autotuned tiles, one graph launch per batch",
+        "pairs": pairs,
+        "value": round(pairs / dt, 1),
+        "unit": "superpositions/s",
+        "ms_per_step": round(dt * 1e3, 3),
+        "device_ms_per_step": round(dev_ms, 3),
+        "first_call_ms_incl_graph_build": round(first_ms, 2),
+        "autotune_s": round(tune_s, 2),
+        "flagged_pairs": flagged,
+        "max_abs_err_vs_fp64_8_ensembles": err,
+        "plain_fp32_ms_per_step": round(plain_dt * 1e3, 3),
+        "speedup_vs_plain": round(plain_dt / dt, 2),
+        "host_io_ms_per_call": round(host_dt * 1e3, 1),
+    }
+
+
+def serving_leg(args, B, log) -> dict:
+    """Device-resident libraries of 25,000 conformations in each of L = 64, 128,
+    256, 512; Poisson arrivals at a fixed rate; each request 1-64 queries of one
+    bucket, top-100 lowest cRMSD."""
+    buckets = (64, 128, 256, 512)
+    n_lib = args.serve_library
+    ctx = B.Context(0)
+    libs, pools = [], []
+    for b, L in enumerate(buckets):
+        x = B.generate_ensemble(8000 + b, n_lib + 256, L, sigma=(0.5, 3.0))
+        libs.append(B.Library(x[:n_lib], ctx=ctx))
+        pools.append(np.ascontiguousarray(x[n_lib:]))
+    log(f"serving: libraries ready ({len(buckets)} x {n_lib})")
+    srv = B.Server(libs, max_inflight=4)
+    rng = np.random.default_rng(11)
+    R = args.serve_requests
+    gaps = rng.exponential(1.0 / args.serve_qps, R)
+    reqs = []
+    for _ in range(R):
+        b = int(rng.integers(0, len(buckets)))
+        nq = int(rng.integers(1, 65))
+        s0 = int(rng.integers(0, 256 - nq + 1))
+        reqs.append((b, pools[b][s0:s0 + nq]))
+    # warm the server's slots and kernels
+    for b in range(len(buckets)):
+        srv.submit(b, pools[b][:64], 100).wait()
+    tickets = []
+    t_start = time.perf_counter()
+    due = t_start
+    for (b, q), g in zip(reqs, gaps):
+        due += g
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            if due - now > 0.002:
+                time.sleep(due - now - 0.001)
+        tickets.append(srv.submit(b, q, 100))
+    for t in tickets:
+        t.wait()
+    wall = time.perf_counter() - t_start
+    lat = np.array([t.latency_ms() for t in tickets])
+    pairs = sum(q.shape[0] for _, q in reqs) * n_lib
+    srv.close()
+    for l in libs:
+        l.close()
+    ctx.close()
+    log(f"serving: p50 {np.percentile(lat, 50):.3f} ms, p99 {np.percentile(lat, 99):.3f} ms")
+    return {
+        "workload": f"libraries of {n_lib} conformations at L = 64/128/256/512 (resident in HBM); "
+                    f"Poisson arrivals at {args.serve_qps} requests/s; 1-64 queries of one bucket "
+                    "per request; top-100",
+        "requests": R,
+        "qps_offered": args.serve_qps,
+        "qps_achieved": round(R / wall, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(lat.mean()), 3),
+        "pairs_per_s": round(pairs / wall, 1),
+        "latency": "submit to host-visible top-100 (bs_server_latency_ms)",
+    }
+
+
 def cpu_baseline(x: np.ndarray, target_s: float, threads: int) -> dict:
     """Times the FP64 oracle on rows [0, R) with R sized for ~target_s seconds."""
     from oracle import oracle as O
@@ -85,6 +228,12 @@ def main() -> None:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-ensembles", type=int, default=512)
+    ap.add_argument("--serve-library", type=int, default=25000)
+    ap.add_argument("--serve-qps", type=float, default=1000.0)
+    ap.add_argument("--serve-requests", type=int, default=3000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="copy a device buffer of this size once before the warmup (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -210,6 +359,9 @@ def main() -> None:
                       f"FP64 QCP oracle, {cb['seconds']:.1f} s",
         }
 
+    batch = None if args.no_batch else batch_leg(args, B, torch, log)
+    serving = None if args.no_serving else serving_leg(args, B, log)
+
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -242,6 +394,8 @@ def main() -> None:
         "error_vs_fp64": err,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -35,7 +35,7 @@ static int choose_tile(const bs_context* ctx, bs_method m, int n, int L)
 
 void BatchWorkspace::bind(Budget* b)
 {
-    for (DevBuf* d : {&xyz, &out, &centred, &rot, &ref, &partial, &panels, &consts, &bits, &meta})
+    for (DevBuf* d : {&xyz, &out, &cent, &rot, &ref, &partial, &panels, &consts, &bits, &meta})
         d->bind(b);
 }
 
@@ -47,7 +47,7 @@ void BatchWorkspace::release()
     exec_key.clear();
     meta_last.clear();
     meta_last_dev = nullptr;
-    for (DevBuf* d : {&xyz, &out, &centred, &rot, &ref, &partial, &panels, &consts, &bits, &meta})
+    for (DevBuf* d : {&xyz, &out, &cent, &rot, &ref, &partial, &panels, &consts, &bits, &meta})
         d->reset();
     if (h_meta) (void)hipHostFree(h_meta);
     if (h_io) (void)hipHostFree(h_io);
@@ -291,10 +291,10 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
 
     // device arrays of the batch
     const int64_t total_conf = P.conf_prefix[E];
-    double* centred = static_cast<double*>(W.centred.ensure((size_t)P.xyz_doubles * 8, "batch centred"));
+    double* cent = static_cast<double*>(W.cent.ensure((size_t)total_conf * 3 * 8, "batch centroids"));
+    double* part = static_cast<double*>(W.partial.ensure((size_t)std::max<int64_t>(P.part_doubles, 1) * 8, "batch partial sums"));
     double* rot = static_cast<double*>(W.rot.ensure((size_t)total_conf * 9 * 8, "batch rotations"));
     double* ref = static_cast<double*>(W.ref.ensure((size_t)P.ref_doubles * 8, "batch references"));
-    double* part = static_cast<double*>(W.partial.ensure((size_t)std::max<int64_t>(P.part_doubles, 1) * 8, "batch partial sums"));
     char* pan = static_cast<char*>(W.panels.ensure((size_t)P.pan_bytes, "batch panels"));
     double* con = static_cast<double*>(W.consts.ensure((size_t)P.con_doubles * 8, "batch constants"));
     const int64_t total_words = P.word_prefix[E];
@@ -305,7 +305,7 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
         BatchEns& b = ens[e];
         const int64_t c0 = P.conf_prefix[e];
         b.xyz = d_xyz[e];
-        b.centred = centred + P.xyz_off[e];
+        b.cent = cent + c0 * 3;
         b.rot = rot + c0 * 9;
         b.ref = ref + P.ref_off[e];
         b.partial = part + P.part_off[e];
@@ -386,7 +386,7 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
         hipGraph_t graph = nullptr;
         BS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            launch_batch_prep(a, rot, s);
+            launch_batch_prep(a, s);
             launch_batch_gram(a, bits, total_words + 16, s);
             launch_batch_fallback(a, s);
         } catch (...) {

@@ -22,7 +22,7 @@ int default_tile(int n, int L);
 // Device and pinned host memory of the batched pipeline, plus the
 // instantiated graph of the last batch shape.
 struct BatchWorkspace {
-    DevBuf xyz, out, centred, rot, ref, partial, panels, consts, bits, meta;
+    DevBuf xyz, out, cent, rot, ref, partial, panels, consts, bits, meta;
     void* h_meta = nullptr;  // pinned: descriptors, prefix sums, work lists
     size_t h_meta_cap = 0;
     void* h_io = nullptr;    // pinned: staging of host inputs and outputs

@@ -18,7 +18,7 @@ size_t prepared_bytes(int n, int L)
     const size_t n_pad = (size_t)round_up(n, kTileRows);
     const size_t n_kblk = (size_t)(L + kAtomBlock - 1) / kAtomBlock;
     const size_t slices = (size_t)(n + kMeanSliceConfs - 1) / kMeanSliceConfs;
-    return (size_t)n * L * 3 * 8 + (size_t)n * 9 * 8 + (size_t)(3 * L + 6) * 8 +
+    return (size_t)n * 3 * 8 + (size_t)n * 9 * 8 + (size_t)(3 * L + 6) * 8 +
            slices * 3 * L * 8 + n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16 +
            (size_t)kNumConsts * n_pad * 8;
 }
@@ -31,26 +31,23 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
     P.n_pad = round_up(n, kTileRows);
     P.n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
     P.mode = mode;
-    const int slices = (n + kMeanSliceConfs - 1) / kMeanSliceConfs;
-    double* centred = static_cast<double*>(P.centred.ensure((size_t)n * L * 3 * 8, "centred coordinates"));
+    P.xyz = d_xyz;
+    double* cent = static_cast<double*>(P.cent.ensure((size_t)n * 3 * 8, "centroids"));
     double* rot = static_cast<double*>(P.rot.ensure((size_t)n * 9 * 8, "rotations"));
     double* ref = static_cast<double*>(P.ref.ensure((size_t)(3 * L + 6) * 8, "reference"));
-    double* partial = static_cast<double*>(P.partial.ensure((size_t)slices * 3 * L * 8, "reference partial sums"));
     void* panels = P.panels.ensure((size_t)P.n_pad / kConfBlock * P.n_kblk * kPanelBlockU4 * 16,
                                    "operand panels");
     double* consts = static_cast<double*>(P.consts.ensure((size_t)kNumConsts * P.n_pad * 8, "constants"));
 
     if (fixed_ref) {
-        launch_centre(d_xyz, n, L, centred, s);
         BS_HIP(hipMemcpyAsync(ref, fixed_ref, (size_t)(3 * L + 6) * 8, hipMemcpyDeviceToDevice, s));
-        launch_fit(centred, n, L, ref, rot, s);
     } else {
-        // fit onto conformation 0, one Procrustes-mean step, fit onto the mean
-        launch_centre_fit0(d_xyz, n, L, centred, rot, s);
-        launch_mean(centred, rot, n, L, partial, ref, s);
-        launch_fit(centred, n, L, ref, rot, s);
+        const size_t slices = (size_t)(n + kMeanSliceConfs - 1) / kMeanSliceConfs;
+        double* partial = static_cast<double*>(
+            P.partial.ensure(slices * 3 * L * 8, "reference partial sums"));
+        launch_reference(d_xyz, n, L, rot, cent, partial, ref, s);
     }
-    launch_pack(centred, rot, ref, n, P.n_pad, L, P.n_kblk, mode, panels, consts, P.n_pad, s);
+    launch_fit_pack(d_xyz, ref, n, P.n_pad, L, P.n_kblk, mode, rot, cent, panels, consts, P.n_pad, s);
     BS_HIP(hipGetLastError());
 }
 
@@ -172,8 +169,8 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     if (rec) rec->stop(kRecMain, s);
     BS_HIP(hipGetLastError());
     if (rec) rec->start(kRecFallback, s);
-    launch_fallback(A.centred.as<double>(), B.centred.as<double>(), A.L, flags, (int64_t)n_out,
-                    g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, s);
+    launch_fallback(A.xyz, A.cent.as<double>(), B.xyz, B.cent.as<double>(), A.L, flags,
+                    (int64_t)n_out, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, s);
     if (rec) rec->stop(kRecFallback, s);
     BS_HIP(hipGetLastError());
     if (rec && rec->h_count)

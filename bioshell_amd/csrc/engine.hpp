@@ -11,23 +11,27 @@
 namespace bs {
 
 // Conformations of one atom count, prepared on the device for the Gram path:
-// centred FP64 copies (for FP64 re-evaluation), the rotation onto the common
-// reference, the reference itself (+ R0), operand panels and constants.
+// the raw coordinates they came from (borrowed: they must outlive every run
+// that reads this set, for the FP64 re-evaluation), centroids, rotations
+// onto the common reference, the reference itself (+ R0), operand panels and
+// constants.
 struct Prepared {
     int n = 0, n_pad = 0, L = 0, n_kblk = 0;
     GramMode mode = kGramBF16x3;
-    DevBuf centred, rot, ref, partial, panels, consts;
+    const double* xyz = nullptr;
+    DevBuf cent, rot, ref, partial, panels, consts;
     void bind(Budget* b)
     {
-        centred.bind(b); rot.bind(b); ref.bind(b); partial.bind(b); panels.bind(b); consts.bind(b);
+        cent.bind(b); rot.bind(b); ref.bind(b); partial.bind(b); panels.bind(b); consts.bind(b);
     }
     void reset()
     {
-        centred.reset(); rot.reset(); ref.reset(); partial.reset(); panels.reset(); consts.reset();
+        cent.reset(); rot.reset(); ref.reset(); partial.reset(); panels.reset(); consts.reset();
+        xyz = nullptr;
     }
     size_t bytes() const
     {
-        return centred.size() + rot.size() + ref.size() + partial.size() + panels.size() +
+        return cent.size() + rot.size() + ref.size() + partial.size() + panels.size() +
                consts.size();
     }
 };
@@ -35,10 +39,11 @@ struct Prepared {
 // Bytes prepare() will hold for n conformations of L atoms.
 size_t prepared_bytes(int n, int L);
 
-// Centre, fit, (reference mean), pack.  If `fixed_ref` (device, 3L + 6
+// Reference, fit, pack (prep_device.hpp).  If `fixed_ref` (device, 3L + 6
 // doubles, the layout of Prepared::ref) is given, conformations are fitted
 // onto it and it is copied as this set's reference; otherwise the reference is
 // one Procrustes-mean step from conformation 0.
+// d_xyz is borrowed (Prepared::xyz).
 void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
              const double* fixed_ref, hipStream_t s);
 

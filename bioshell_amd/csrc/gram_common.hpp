@@ -89,16 +89,19 @@ __device__ inline int tri_row(int64_t o, int n)
     return i;
 }
 
-// FP64 cRMSD of one pair from centred coordinates x, y (L atoms each), by one
-// wave: the 64 lanes split the atoms, then the textbook QCP with the
-// near-degenerate polish.  Every lane returns the result.
-__device__ inline double fallback_pair(const double* __restrict__ x, const double* __restrict__ y,
+// FP64 cRMSD of one pair from raw coordinates x, y (L atoms each) and their
+// centroids cx, cy, by one wave: the 64 lanes split the atoms, centring on the
+// fly, then the textbook QCP with the near-degenerate polish.  Every lane
+// returns the result.
+__device__ inline double fallback_pair(const double* __restrict__ x, const double* __restrict__ cx,
+                                       const double* __restrict__ y, const double* __restrict__ cy,
                                        int L, int lane)
 {
+    const double x0 = cx[0], x1 = cx[1], x2 = cx[2], y0 = cy[0], y1 = cy[1], y2 = cy[2];
     double m9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ga = 0.0, gb = 0.0;
     for (int k = lane; k < L; k += 64) {
-        const double a0 = x[3 * k], a1 = x[3 * k + 1], a2 = x[3 * k + 2];
-        const double b0 = y[3 * k], b1 = y[3 * k + 1], b2 = y[3 * k + 2];
+        const double a0 = x[3 * k] - x0, a1 = x[3 * k + 1] - x1, a2 = x[3 * k + 2] - x2;
+        const double b0 = y[3 * k] - y0, b1 = y[3 * k + 1] - y1, b2 = y[3 * k + 2] - y2;
         m9[0] += a0 * b0; m9[1] += a0 * b1; m9[2] += a0 * b2;
         m9[3] += a1 * b0; m9[4] += a1 * b1; m9[5] += a1 * b2;
         m9[6] += a2 * b0; m9[7] += a2 * b1; m9[8] += a2 * b2;

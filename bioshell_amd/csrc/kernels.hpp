@@ -32,27 +32,20 @@ constexpr int kConfBlock = 32;
 constexpr int kAtomBlock = 16;
 constexpr int kPanelBlockU4 = 384;  // 16-byte units per (conf block, atom block)
 
-constexpr int kMeanSliceConfs = 32;  // conformations per partial sum of the reference mean
-
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// Centre every conformation (sum, divide, subtract).
-void launch_centre(const double* xyz, int n, int L, double* centred, hipStream_t s);
-// Centring fused with the first reference fit (onto conformation 0).
-void launch_centre_fit0(const double* xyz, int n, int L, double* centred, double* rot,
-                        hipStream_t s);
-// Reference fit: rotation of each centred conformation onto ref (L x 3).
-void launch_fit(const double* centred, int n, int L, const double* ref, double* rot,
-                hipStream_t s);
-// ref_out = mean over conformations of rot_i * centred_i; then R0 after it.
-void launch_mean(const double* centred, const double* rot, int n, int L, double* partial,
-                 double* ref_out, hipStream_t s);
-// Copy conformation 0 as the initial reference (plus its R0).
-void launch_ref_from_first(const double* centred, int L, double* ref, hipStream_t s);
-// Residuals e_i = rot_i x_i - ref, split into operand panels, plus constants.
-void launch_pack(const double* centred, const double* rot, const double* ref, int n, int n_pad,
-                 int L, int n_kblk, GramMode mode, void* panels, double* consts, int ld,
-                 hipStream_t s);
+constexpr int kMeanSliceConfs = 32;  // conformations per partial sum of the reference mean
+
+// Reference of one ensemble (prep_device.hpp): centroids (cent, n x 3) and
+// rotations onto conformation 0 (rot, n x 9) as by-products, the slices'
+// partial sums (partial, ceil(n/32) x 3L), then r (ref: 3L values) and R0 (6).
+void launch_reference(const double* xyz, int n, int L, double* rot, double* cent, double* partial,
+                      double* ref, hipStream_t s);
+// Fit onto ref and pack: rotations (rot), centroids (cent), operand panels of
+// e_i = rot_i (x_i - c_i) - ref and the constants, for n_pad rows.
+void launch_fit_pack(const double* xyz, const double* ref, int n, int n_pad, int L, int n_kblk,
+                     GramMode mode, double* rot, double* cent, void* panels, double* consts, int ld,
+                     hipStream_t s);
 
 struct GramArgs {
     const void* panels_a;   // row operand panels
@@ -91,10 +84,12 @@ void launch_gram_pipe(const GramArgs& a, GramMode mode, hipStream_t s);
 void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
 // FP64 re-evaluation of the outputs whose bit is set in `bits` (n_out bits),
-// from centred coordinates; `count` (may be null) accumulates how many.
-void launch_fallback(const double* centred_a, const double* centred_b, int L,
-                     const uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld,
-                     void* out, int out_f64, uint32_t* count, hipStream_t s);
+// from raw coordinates and centroids of rows (a) and columns (b); `count`
+// (may be null) accumulates how many.
+void launch_fallback(const double* xyz_a, const double* cent_a, const double* xyz_b,
+                     const double* cent_b, int L, const uint32_t* bits, int64_t n_out,
+                     int triangle, int n, int64_t out_ld, void* out, int out_f64, uint32_t* count,
+                     hipStream_t s);
 void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s);
 
 // Per-query top-k over a dense n_q x n_lib score matrix (ascending, ties by index).
@@ -106,8 +101,8 @@ void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, 
 // Per-conformation arrays (centred, rot) are slices of batch-wide arrays.
 struct BatchEns {
     const double* xyz;      // n x L x 3 f64 input
-    double* centred;        // n x L x 3
-    double* rot;            // n x 9 (covariance, then rotation)
+    double* cent;           // n x 3 centroids
+    double* rot;            // n x 9 rotations
     double* ref;            // 3L + 6
     double* partial;        // n_slices x 3L
     void* panels;           // n_pad / 32 x n_kblk blocks
@@ -115,7 +110,7 @@ struct BatchEns {
     uint32_t* bits;         // flag bitmap of the n(n-1)/2 outputs
     float* out;             // packed triangle, f32
     int n, L, n_pad, n_kblk;
-    int n_slices, tile;     // tile: 32 or 64
+    int n_slices, tile;     // reference-mean slices; tile side 32 or 64
     double acc_coef, rep_g_max;  // error-model thresholds at this L
 };
 
@@ -150,7 +145,7 @@ struct BatchArgs {
 };
 
 // The batched pipeline's launches, in order (each over all ensembles).
-void launch_batch_prep(const BatchArgs& b, double* rot_all, hipStream_t s);
+void launch_batch_prep(const BatchArgs& b, hipStream_t s);
 void launch_batch_gram(const BatchArgs& b, uint32_t* zero_base, int64_t zero_words, hipStream_t s);
 void launch_batch_fallback(const BatchArgs& b, hipStream_t s);
 

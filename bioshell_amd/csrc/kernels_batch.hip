@@ -1,7 +1,7 @@
 // kernels_batch.hip -- many ensembles of uneven size (n conformations, L
 // atoms each) in a fixed sequence of launches, independent of how many
 // ensembles there are: every preparation step is one launch over all the
-// batch's conformations (or slices, coordinates, ...), the Gram step is one
+// batch's conformations (or mean slices, coordinates), the Gram step is one
 // persistent launch over a work list of (ensemble, tile) items, and the FP64
 // re-evaluation is one launch over all flag bitmaps.  The host captures the
 // sequence in a HIP graph.
@@ -33,33 +33,25 @@ __device__ inline int find_ens(const T* __restrict__ prefix, int n_ens, T g)
 }
 
 // ------------------------------------------------------------------ prep
-__global__ __launch_bounds__(256) void batch_centre_cov0_kernel(BatchArgs b)
+__global__ __launch_bounds__(256) void batch_fit0_kernel(BatchArgs b)
 {
     const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= b.total_conf) return;
     const int e = find_ens(b.conf_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
     const int i = g - b.conf_prefix[e];
-    centre_one<true>(E.xyz + (int64_t)i * E.L * 3, E.L, E.centred + (int64_t)i * E.L * 3, E.xyz,
-                     E.rot + (int64_t)i * 9, threadIdx.x & 63);
-}
-
-// rot_all is the batch-wide rotation array (ensemble e's slice starts at
-// conformation conf_prefix[e]), so no lookup is needed.
-__global__ __launch_bounds__(64) void batch_rot_kernel(double* __restrict__ rot_all, int total)
-{
-    const int g = blockIdx.x * 64 + threadIdx.x;
-    if (g >= total) return;
-    rot_one(rot_all + (int64_t)g * 9);
+    fit0_one(E.xyz + (int64_t)i * E.L * 3, E.xyz, E.L, E.rot + (int64_t)i * 9,
+             E.cent + (int64_t)i * 3, threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(256) void batch_mean_partial_kernel(BatchArgs b)
 {
     __shared__ double sR[kMeanSliceConfs][9];
+    __shared__ double sC[kMeanSliceConfs][3];
     const int g = blockIdx.x;
     const int e = find_ens(b.slice_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    mean_partial_block(E.centred, E.rot, E.n, E.L, g - b.slice_prefix[e], E.partial, sR);
+    mean_partial_block(E.xyz, E.cent, E.rot, E.n, E.L, g - b.slice_prefix[e], E.partial, sR, sC);
 }
 
 __global__ __launch_bounds__(256) void batch_mean_final_kernel(BatchArgs b)
@@ -79,19 +71,8 @@ __global__ __launch_bounds__(256) void batch_ref_gram_kernel(BatchArgs b)
     ref_gram_block(E.ref, E.L, part);
 }
 
-__global__ __launch_bounds__(256) void batch_fit_cov_kernel(BatchArgs b)
-{
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= b.total_conf) return;
-    const int e = find_ens(b.conf_prefix, b.n_ens, g);
-    const BatchEns& E = b.ens[e];
-    const int i = g - b.conf_prefix[e];
-    fit_cov_one(E.centred + (int64_t)i * E.L * 3, E.L, E.ref, E.rot + (int64_t)i * 9,
-                threadIdx.x & 63);
-}
-
 template <int MODE>
-__global__ __launch_bounds__(256) void batch_pack_kernel(BatchArgs b)
+__global__ __launch_bounds__(256) void batch_fit_pack_kernel(BatchArgs b)
 {
     __shared__ uint4 img[4][kPackWordsU4];
     const int wv = threadIdx.x >> 6;
@@ -99,8 +80,9 @@ __global__ __launch_bounds__(256) void batch_pack_kernel(BatchArgs b)
     if (g >= b.total_pad) return;
     const int e = find_ens(b.pad_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    pack_one<MODE>(E.centred, E.rot, E.ref, E.n, E.L, E.n_kblk, g - b.pad_prefix[e],
-                   static_cast<uint4*>(E.panels), E.consts, E.n_pad, img[wv], threadIdx.x & 63);
+    fit_pack_one<MODE>(E.xyz, E.ref, E.n, E.L, E.n_kblk, g - b.pad_prefix[e], E.rot, E.cent,
+                       static_cast<uint4*>(E.panels), E.consts, E.n_pad, img[wv],
+                       threadIdx.x & 63);
 }
 
 __global__ void batch_zero_kernel(uint4* p, int64_t n4)
@@ -234,8 +216,9 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(BatchArgs b)
             if (o >= n_out) continue;
             const int i = tri_row(o, E.n);
             const int j = (int)(o - ((int64_t)i * E.n - (int64_t)i * (i + 1) / 2)) + i + 1;
-            const double r = fallback_pair(E.centred + (int64_t)i * E.L * 3,
-                                           E.centred + (int64_t)j * E.L * 3, E.L, lane);
+            const double r = fallback_pair(E.xyz + (int64_t)i * E.L * 3, E.cent + 3 * (int64_t)i,
+                                           E.xyz + (int64_t)j * E.L * 3, E.cent + 3 * (int64_t)j,
+                                           E.L, lane);
             if (lane == 0) E.out[o] = (float)r;
             ++done;
         }
@@ -245,21 +228,18 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(BatchArgs b)
 
 }  // namespace
 
-void launch_batch_prep(const BatchArgs& b, double* rot_all, hipStream_t s)
+void launch_batch_prep(const BatchArgs& b, hipStream_t s)
 {
     const int cw = (b.total_conf + 3) / 4;
-    batch_centre_cov0_kernel<<<cw, 256, 0, s>>>(b);
-    batch_rot_kernel<<<(b.total_conf + 63) / 64, 64, 0, s>>>(rot_all, b.total_conf);
+    batch_fit0_kernel<<<cw, 256, 0, s>>>(b);
     batch_mean_partial_kernel<<<b.total_slices, 256, 0, s>>>(b);
     batch_mean_final_kernel<<<(b.total_coords + 3) / 4, 256, 0, s>>>(b);
     batch_ref_gram_kernel<<<b.n_ens, 256, 0, s>>>(b);
-    batch_fit_cov_kernel<<<cw, 256, 0, s>>>(b);
-    batch_rot_kernel<<<(b.total_conf + 63) / 64, 64, 0, s>>>(rot_all, b.total_conf);
     const int pw = (b.total_pad + 3) / 4;
     if (b.mode == kGramBF16x3)
-        batch_pack_kernel<kGramBF16x3><<<pw, 256, 0, s>>>(b);
+        batch_fit_pack_kernel<kGramBF16x3><<<pw, 256, 0, s>>>(b);
     else
-        batch_pack_kernel<kGramFP32><<<pw, 256, 0, s>>>(b);
+        batch_fit_pack_kernel<kGramFP32><<<pw, 256, 0, s>>>(b);
 }
 
 void launch_batch_gram(const BatchArgs& b, uint32_t* zero_base, int64_t zero_words, hipStream_t s)

@@ -60,8 +60,10 @@ __global__ void zero_words_kernel(uint4* p, int64_t n4)
 // FP64 re-evaluation of flagged outputs: each wave takes 64 bitmap words (one
 // per lane) and processes their set bits one pair at a time, the 64 lanes
 // splitting the atoms; textbook QCP with the near-degenerate polish.
-__global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ ca,
-                                                       const double* __restrict__ cbp, int L,
+__global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ xa,
+                                                       const double* __restrict__ ca,
+                                                       const double* __restrict__ xb,
+                                                       const double* __restrict__ cb, int L,
                                                        const uint32_t* __restrict__ bits,
                                                        int64_t n_out, int triangle, int n,
                                                        int64_t out_ld, void* __restrict__ out,
@@ -92,7 +94,8 @@ __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict_
                 i = (int)(o / out_ld);
                 j = (int)(o - (int64_t)i * out_ld);
             }
-            const double r = fallback_pair(ca + (int64_t)i * L * 3, cbp + (int64_t)j * L * 3, L, lane);
+            const double r = fallback_pair(xa + (int64_t)i * L * 3, ca + 3 * (int64_t)i,
+                                           xb + (int64_t)j * L * 3, cb + 3 * (int64_t)j, L, lane);
             if (lane == 0) {
                 if (out_f64)
                     static_cast<double*>(out)[o] = r;
@@ -143,14 +146,15 @@ void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s)
     zero_words_kernel<<<(int)blocks, 256, 0, s>>>(reinterpret_cast<uint4*>(p), n4);
 }
 
-void launch_fallback(const double* centred_a, const double* centred_b, int L,
-                     const uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld,
-                     void* out, int out_f64, uint32_t* count, hipStream_t s)
+void launch_fallback(const double* xyz_a, const double* cent_a, const double* xyz_b,
+                     const double* cent_b, int L, const uint32_t* bits, int64_t n_out,
+                     int triangle, int n, int64_t out_ld, void* out, int out_f64, uint32_t* count,
+                     hipStream_t s)
 {
     const int64_t n_words = (n_out + 31) / 32;
     const int64_t blocks = std::min<int64_t>(2048, std::max<int64_t>(1, (n_words + 255) / 256));
-    fallback_kernel<<<(int)blocks, 256, 0, s>>>(centred_a, centred_b, L, bits, n_out, triangle, n,
-                                                out_ld, out, out_f64, count);
+    fallback_kernel<<<(int)blocks, 256, 0, s>>>(xyz_a, cent_a, xyz_b, cent_b, L, bits, n_out,
+                                                triangle, n, out_ld, out, out_f64, count);
 }
 
 }  // namespace bs

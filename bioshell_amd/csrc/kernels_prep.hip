@@ -1,55 +1,32 @@
 // kernels_prep.hip -- single-ensemble launches of the preparation steps
-// (bodies in prep_device.hpp) and the single-pair superposition kernel.
+// (bodies and the data flow in prep_device.hpp) and the single-pair
+// superposition kernel.
 #include "prep_device.hpp"
 
 namespace bs {
 namespace {
 
-__global__ __launch_bounds__(256) void centre_kernel(const double* __restrict__ xyz, int n, int L,
-                                                     double* __restrict__ out)
+// Reference: centroid and fit onto conformation 0 for every conformation
+// (wave each), slices of the rotated sum, the mean (wave per coordinate),
+// then R0 (one block).
+__global__ __launch_bounds__(256) void fit0_kernel(const double* __restrict__ xyz, int n, int L,
+                                                   double* __restrict__ rot,
+                                                   double* __restrict__ cent)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    centre_one<false>(xyz + (int64_t)i * L * 3, L, out + (int64_t)i * L * 3, nullptr, nullptr,
-                      threadIdx.x & 63);
+    fit0_one(xyz + (int64_t)i * L * 3, xyz, L, rot + (int64_t)i * 9, cent + (int64_t)i * 3,
+             threadIdx.x & 63);
 }
 
-// Centring fused with the covariance against conformation 0 (the first fit).
-__global__ __launch_bounds__(256) void centre_cov0_kernel(const double* __restrict__ xyz, int n,
-                                                          int L, double* __restrict__ out,
-                                                          double* __restrict__ cov)
-{
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    centre_one<true>(xyz + (int64_t)i * L * 3, L, out + (int64_t)i * L * 3, xyz, cov + (int64_t)i * 9,
-                     threadIdx.x & 63);
-}
-
-// Fit in two stages: one wave per conformation reduces the covariance with
-// the reference; then one thread per conformation solves Horn's 4x4
-// eigenproblem and overwrites its 9 doubles with the rotation.
-__global__ __launch_bounds__(256) void fit_cov_kernel(const double* __restrict__ centred, int n,
-                                                      int L, const double* __restrict__ ref,
-                                                      double* __restrict__ cov)
-{
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    fit_cov_one(centred + (int64_t)i * L * 3, L, ref, cov + (int64_t)i * 9, threadIdx.x & 63);
-}
-
-__global__ __launch_bounds__(64) void fit_rot_kernel(double* __restrict__ rot, int n)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= n) return;
-    rot_one(rot + (int64_t)i * 9);
-}
-
-__global__ __launch_bounds__(256) void mean_partial_kernel(const double* __restrict__ centred,
+__global__ __launch_bounds__(256) void mean_partial_kernel(const double* __restrict__ xyz,
+                                                           const double* __restrict__ cent,
                                                            const double* __restrict__ rot, int n,
                                                            int L, double* __restrict__ partial)
 {
     __shared__ double sR[kMeanSliceConfs][9];
-    mean_partial_block(centred, rot, n, L, blockIdx.x, partial, sR);
+    __shared__ double sC[kMeanSliceConfs][3];
+    mean_partial_block(xyz, cent, rot, n, L, blockIdx.x, partial, sR, sC);
 }
 
 __global__ __launch_bounds__(256) void mean_final_kernel(const double* __restrict__ partial,
@@ -67,26 +44,21 @@ __global__ __launch_bounds__(256) void ref_gram_kernel(double* __restrict__ ref,
     ref_gram_block(ref, L, part);
 }
 
-__global__ void copy_kernel(const double* __restrict__ src, double* __restrict__ dst, int count)
-{
-    const int v = blockIdx.x * 256 + threadIdx.x;
-    if (v < count) dst[v] = src[v];
-}
-
 template <int MODE>
-__global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ centred,
-                                                   const double* __restrict__ rot,
-                                                   const double* __restrict__ ref, int n,
-                                                   int n_pad, int L, int n_kblk,
-                                                   uint4* __restrict__ panels,
-                                                   double* __restrict__ consts, int ld)
+__global__ __launch_bounds__(256) void fit_pack_kernel(const double* __restrict__ xyz,
+                                                       const double* __restrict__ ref, int n,
+                                                       int n_pad, int L, int n_kblk,
+                                                       double* __restrict__ rot,
+                                                       double* __restrict__ cent,
+                                                       uint4* __restrict__ panels,
+                                                       double* __restrict__ consts, int ld)
 {
     __shared__ uint4 img[4][kPackWordsU4];
     const int wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 4 + wv;  // grid covers n_pad exactly
     if (i >= n_pad) return;
-    pack_one<MODE>(centred, rot, ref, n, L, n_kblk, i, panels, consts, ld, img[wv],
-                   threadIdx.x & 63);
+    fit_pack_one<MODE>(xyz, ref, n, L, n_kblk, i, rot, cent, panels, consts, ld, img[wv],
+                       threadIdx.x & 63);
 }
 
 // ------------------------------------------------------ single superposition
@@ -134,50 +106,28 @@ __global__ __launch_bounds__(64) void superpose_kernel(const double* __restrict_
 
 }  // namespace
 
-void launch_centre(const double* xyz, int n, int L, double* centred, hipStream_t s)
-{
-    centre_kernel<<<(n + 3) / 4, 256, 0, s>>>(xyz, n, L, centred);
-}
-
-void launch_centre_fit0(const double* xyz, int n, int L, double* centred, double* rot,
-                        hipStream_t s)
-{
-    centre_cov0_kernel<<<(n + 3) / 4, 256, 0, s>>>(xyz, n, L, centred, rot);
-    fit_rot_kernel<<<(n + 63) / 64, 64, 0, s>>>(rot, n);
-}
-
-void launch_fit(const double* centred, int n, int L, const double* ref, double* rot, hipStream_t s)
-{
-    fit_cov_kernel<<<(n + 3) / 4, 256, 0, s>>>(centred, n, L, ref, rot);
-    fit_rot_kernel<<<(n + 63) / 64, 64, 0, s>>>(rot, n);
-}
-
-void launch_mean(const double* centred, const double* rot, int n, int L, double* partial,
-                 double* ref_out, hipStream_t s)
+void launch_reference(const double* xyz, int n, int L, double* rot, double* cent, double* partial,
+                      double* ref, hipStream_t s)
 {
     const int slices = (n + kMeanSliceConfs - 1) / kMeanSliceConfs;
-    mean_partial_kernel<<<slices, 256, 0, s>>>(centred, rot, n, L, partial);
-    mean_final_kernel<<<(3 * L + 3) / 4, 256, 0, s>>>(partial, slices, n, L, ref_out);
-    ref_gram_kernel<<<1, 256, 0, s>>>(ref_out, L);
-}
-
-void launch_ref_from_first(const double* centred, int L, double* ref, hipStream_t s)
-{
-    copy_kernel<<<(3 * L + 255) / 256, 256, 0, s>>>(centred, ref, 3 * L);
+    fit0_kernel<<<(n + 3) / 4, 256, 0, s>>>(xyz, n, L, rot, cent);
+    mean_partial_kernel<<<slices, 256, 0, s>>>(xyz, cent, rot, n, L, partial);
+    mean_final_kernel<<<(3 * L + 3) / 4, 256, 0, s>>>(partial, slices, n, L, ref);
     ref_gram_kernel<<<1, 256, 0, s>>>(ref, L);
 }
 
-void launch_pack(const double* centred, const double* rot, const double* ref, int n, int n_pad,
-                 int L, int n_kblk, GramMode mode, void* panels, double* consts, int ld,
-                 hipStream_t s)
+void launch_fit_pack(const double* xyz, const double* ref, int n, int n_pad, int L, int n_kblk,
+                     GramMode mode, double* rot, double* cent, void* panels, double* consts, int ld,
+                     hipStream_t s)
 {
     const int blocks = (n_pad + 3) / 4;
+    uint4* p = static_cast<uint4*>(panels);
     if (mode == kGramBF16x3)
-        pack_kernel<kGramBF16x3><<<blocks, 256, 0, s>>>(centred, rot, ref, n, n_pad, L, n_kblk,
-                                                        static_cast<uint4*>(panels), consts, ld);
+        fit_pack_kernel<kGramBF16x3><<<blocks, 256, 0, s>>>(xyz, ref, n, n_pad, L, n_kblk, rot,
+                                                            cent, p, consts, ld);
     else
-        pack_kernel<kGramFP32><<<blocks, 256, 0, s>>>(centred, rot, ref, n, n_pad, L, n_kblk,
-                                                      static_cast<uint4*>(panels), consts, ld);
+        fit_pack_kernel<kGramFP32><<<blocks, 256, 0, s>>>(xyz, ref, n, n_pad, L, n_kblk, rot, cent,
+                                                          p, consts, ld);
 }
 
 void launch_superpose(const double* a, const double* b, int L, double* out13, hipStream_t s)

@@ -103,9 +103,10 @@ bs_status bs_library_create(bs_context* ctx, const double* xyz, size_t n_conf, s
         const size_t bytes = n_conf * n_atoms * 3 * 8;
         double* d = static_cast<double*>(lib->staging.ensure(bytes, "library coordinates"));
         BS_HIP(hipMemcpyAsync(d, xyz, bytes, hipMemcpyHostToDevice, s));
+        // the coordinates stay resident beside the prepared set: the FP64
+        // re-evaluation of flagged query pairs reads them
         prepare(lib->P, d, (int)n_conf, (int)n_atoms, kGramBF16x3, nullptr, s);
         BS_HIP(hipStreamSynchronize(s));
-        lib->staging.reset();  // only the prepared set stays resident
         lib->P.partial.reset();
         *out = lib.release();
     });
@@ -117,6 +118,7 @@ void bs_library_destroy(bs_library* lib)
     (void)guard([&] {
         BS_HIP(hipSetDevice(lib->ctx->device));
         lib->P.reset();
+        lib->staging.reset();
     });
     delete lib;
 }

@@ -9,7 +9,7 @@
 struct bs_library {
     bs_context* ctx = nullptr;
     bs::Prepared P;       // prepared conformations (own reference)
-    bs::DevBuf staging;   // raw coordinates, only during creation
+    bs::DevBuf staging;   // raw coordinates (read by the FP64 re-evaluation)
 };
 
 namespace bs {

@@ -1,5 +1,6 @@
 // testhooks.cpp -- host builds of the device QCP routines, exported for CPU
 // tests of the algebra (not part of the public ABI in include/bioshell_gpu.h).
+#include "prep_device.hpp"
 #include "qcp.hpp"
 
 extern "C" {
@@ -19,6 +20,10 @@ double bsi_qcp_rmsd_f64(const double* M, double Ga, double Gb, int n)
 {
     return bs::qcp_rmsd<double>(M, Ga, Gb, n);
 }
+
+// Rotation (row-major) that moves b onto a, from M = sum b a^T, as the
+// preparation kernels compute it.
+void bsi_rotation_fast(const double* M, double* R) { bs::rotation_fast(M, R); }
 
 float bsi_qcp_rmsd_f32(const float* M, float Ga, float Gb, int n)
 {

@@ -169,3 +169,48 @@ def test_host_build_of_shifted_qcp_matches_oracle():
             fn(E.ctypes.data_as(dp), cs[i].ctypes.data_as(dp), cs[j].ctypes.data_as(dp), out.ctypes.data_as(dp))
             assert out[3] == 1.0
             assert np.sqrt(out[0] / 50) == pytest.approx(O.crmsd(x[i], x[j]), abs=1e-8)
+
+
+def test_host_build_of_fast_rotation():
+    """The preparation's rotation (Newton on the key quartic + adjugate column)
+    is a proper rotation, optimal for well-posed sets (its fit RMSD equals the
+    oracle's cRMSD), and still a proper rotation for degenerate sets."""
+    from oracle import oracle as O
+
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn = _hook("bsi_rotation_fast", None, [dp, dp])
+    rng = np.random.default_rng(5)
+
+    def rand_rot(g):
+        q = g.normal(size=4)
+        q /= np.linalg.norm(q)
+        a, b, c, d = q
+        return np.array([[a*a+b*b-c*c-d*d, 2*(b*c-a*d), 2*(b*d+a*c)],
+                         [2*(b*c+a*d), a*a-b*b+c*c-d*d, 2*(c*d-a*b)],
+                         [2*(b*d-a*c), 2*(c*d+a*b), a*a-b*b-c*c+d*d]])
+
+    def rot_of(a, b):
+        ac, bc = a - a.mean(0), b - b.mean(0)
+        M = np.ascontiguousarray(bc.T @ ac)  # sum b a^T
+        R = np.empty(9)
+        fn(M.ravel().ctypes.data_as(dp), R.ctypes.data_as(dp))
+        R = R.reshape(3, 3)
+        assert np.abs(R @ R.T - np.eye(3)).max() < 1e-12
+        assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-12)
+        return R, ac, bc
+
+    for L in (3, 4, 10, 76, 300):
+        for _ in range(20):
+            a = rng.normal(size=(L, 3)) * 5
+            b = a @ rand_rot(rng).T + rng.normal(size=(L, 3)) * rng.uniform(0.01, 3.0)
+            R, ac, bc = rot_of(a, b)
+            fit = np.sqrt(((bc @ R.T - ac) ** 2).sum() / L)
+            assert fit == pytest.approx(O.crmsd(a, b), abs=1e-9)
+    # degenerate: M = 0 (one atom), collinear pairs, coplanar sets, identical sets
+    for a, b in [(np.zeros((1, 3)), np.ones((1, 3))),
+                 (np.array([[0.0, 0, 0], [3.8, 0, 0]]), np.array([[1.0, 2, 3], [1.0, 5.8, 3]])),
+                 (np.c_[rng.normal(size=(8, 2)), np.zeros(8)], np.c_[rng.normal(size=(8, 2)), np.zeros(8)]),
+                 (np.arange(12.0).reshape(4, 3) ** 1.5, np.arange(12.0).reshape(4, 3) ** 1.5)]:
+        R, ac, bc = rot_of(a, b)
+        fit = np.sqrt(((bc @ R.T - ac) ** 2).sum() / len(a))
+        assert fit <= O.crmsd(a, b) + 1e-6 or len(a) < 3
