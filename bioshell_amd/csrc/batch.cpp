@@ -298,7 +298,8 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     char* pan = static_cast<char*>(W.panels.ensure((size_t)P.pan_bytes, "batch panels"));
     double* con = static_cast<double*>(W.consts.ensure((size_t)P.con_doubles * 8, "batch constants"));
     const int64_t total_words = P.word_prefix[E];
-    uint32_t* bits = static_cast<uint32_t*>(W.bits.ensure((size_t)(total_words + 16) * 4, "batch flag bitmaps"));
+    uint32_t* bits = static_cast<uint32_t*>(W.bits.ensure(
+        (size_t)(total_words + 16) * 4 + (size_t)kFlagListCap * 8, "batch flag bitmaps and queue"));
 
     std::vector<BatchEns> ens(E);
     for (int e = 0; e < E; ++e) {
@@ -368,7 +369,9 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     a.list_start = reinterpret_cast<const int*>(meta + o_ls);
     a.tiles64 = reinterpret_cast<const uint64_t*>(meta + o_64);
     a.tiles32 = reinterpret_cast<const uint64_t*>(meta + o_32);
+    a.ens_bits0 = bits;
     a.counters = bits + total_words;
+    a.flag_list = reinterpret_cast<int64_t*>(bits + total_words + 16);
     a.kappa_dup = P.th[0].kappa_dup;
     a.u_rep = P.th[0].u_rep;
     a.tau = P.th[0].tau;

@@ -124,14 +124,15 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     }
     // Flagged outputs are marked in a bitmap indexed by output position (one
     // bit per pair: 6.25 MB for 50M pairs), so there is no capacity limit and
-    // no host round trip; the re-evaluation kernel scans it after the Gram
-    // kernel.  The word after the bitmap counts the re-evaluated pairs.
+    // no host round trip; after the Gram kernel the set bits are queued (the
+    // word after the bitmap counts them) and re-evaluated one wave each.
     const uint64_t n_out =
         run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * (uint64_t)run.out_ld;
     const uint64_t n_words = ((n_out + 31) / 32 + 3) / 4 * 4;
-    uint32_t* flags = static_cast<uint32_t*>(
-        sc.flags.ensure((size_t)(n_words + 4) * 4, "flag bitmap"));
+    uint32_t* flags = static_cast<uint32_t*>(sc.flags.ensure(
+        (size_t)(n_words + 4) * 4 + (size_t)kFlagListCap * 8, "flag bitmap and queue"));
     uint32_t* count = flags + n_words;
+    int64_t* list = reinterpret_cast<int64_t*>(flags + n_words + 4);
     launch_zero_words(flags, (int64_t)n_words + 4, s);
 
     GramArgs g{};
@@ -170,7 +171,8 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     BS_HIP(hipGetLastError());
     if (rec) rec->start(kRecFallback, s);
     launch_fallback(A.xyz, A.cent.as<double>(), B.xyz, B.cent.as<double>(), A.L, flags,
-                    (int64_t)n_out, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, s);
+                    (int64_t)n_out, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, list,
+                    kFlagListCap, s);
     if (rec) rec->stop(kRecFallback, s);
     BS_HIP(hipGetLastError());
     if (rec && rec->h_count)

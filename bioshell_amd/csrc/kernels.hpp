@@ -84,12 +84,19 @@ void launch_gram_pipe(const GramArgs& a, GramMode mode, hipStream_t s);
 void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
 // FP64 re-evaluation of the outputs whose bit is set in `bits` (n_out bits),
-// from raw coordinates and centroids of rows (a) and columns (b); `count`
-// (may be null) accumulates how many.
+// from raw coordinates and centroids of rows (a) and columns (b): the set bits
+// are queued into `list` (cap entries; *count, zeroed by the caller, receives
+// the number flagged), one wave re-evaluates each entry, and bits that did not
+// fit in the list are re-evaluated by a scan of the bitmap.
 void launch_fallback(const double* xyz_a, const double* cent_a, const double* xyz_b,
-                     const double* cent_b, int L, const uint32_t* bits, int64_t n_out,
-                     int triangle, int n, int64_t out_ld, void* out, int out_f64, uint32_t* count,
-                     hipStream_t s);
+                     const double* cent_b, int L, uint32_t* bits, int64_t n_out, int triangle,
+                     int n, int64_t out_ld, void* out, int out_f64, uint32_t* count, int64_t* list,
+                     int cap, hipStream_t s);
+// Queues the set bits of bits[0 .. n_words) as bit indices (word * 32 + bit)
+// into list (cap entries), clearing the queued ones; *count += set bits.
+void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,
+                         hipStream_t s);
+constexpr int kFlagListCap = 1 << 20;  // queued FP64 re-evaluations per call (8 MiB)
 void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s);
 
 // Per-query top-k over a dense n_q x n_lib score matrix (ascending, ties by index).
@@ -139,7 +146,9 @@ struct BatchArgs {
     const int* list_start;  // 9 entries: list x is items[list_start[x] .. list_start[x+1])
     const uint64_t* tiles64;
     const uint64_t* tiles32;
+    uint32_t* ens_bits0;    // the batch-wide bitmap (ensemble 0's bitmap first)
     uint32_t* counters;     // 8 list counters, then the flagged-pair count
+    int64_t* flag_list;     // kFlagListCap queued re-evaluations (batch-wide bit indices)
     double kappa_dup, u_rep, tau;
     int n_wg;               // persistent work-groups of the Gram kernel
 };

@@ -54,14 +54,18 @@ __global__ __launch_bounds__(256) void batch_mean_partial_kernel(BatchArgs b)
     mean_partial_block(E.xyz, E.cent, E.rot, E.n, E.L, g - b.slice_prefix[e], E.partial, sR, sC);
 }
 
+// One thread per reference coordinate (batches hold few slices per ensemble;
+// the slices are added in order, as in the single-ensemble wave version).
 __global__ __launch_bounds__(256) void batch_mean_final_kernel(BatchArgs b)
 {
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= b.total_coords) return;
     const int e = find_ens(b.coord_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    mean_final_wave(E.partial, E.n_slices, E.n, E.L, g - b.coord_prefix[e], E.ref,
-                    threadIdx.x & 63);
+    const int v = g - b.coord_prefix[e];
+    double acc = 0.0;
+    for (int s = 0; s < E.n_slices; ++s) acc += E.partial[(int64_t)s * 3 * E.L + v];
+    E.ref[v] = acc / E.n;
 }
 
 __global__ __launch_bounds__(256) void batch_ref_gram_kernel(BatchArgs b)
@@ -189,18 +193,44 @@ __global__ __launch_bounds__(256, 2) void batch_gram_kernel(BatchArgs b)
 }
 
 // ------------------------------------------------------------------ FP64 re-evaluation
-// Each wave takes 64 consecutive words of the batch-wide bitmap (one per
-// lane) and re-evaluates their set bits one pair at a time.
+// Flagged pair of ensemble E at output o: one wave, FP64 textbook QCP.
+__device__ inline void batch_eval_one(const BatchEns& E, int64_t o, int lane)
+{
+    const int i = tri_row(o, E.n);
+    const int j = (int)(o - ((int64_t)i * E.n - (int64_t)i * (i + 1) / 2)) + i + 1;
+    const double r = fallback_pair(E.xyz + (int64_t)i * E.L * 3, E.cent + 3 * (int64_t)i,
+                                   E.xyz + (int64_t)j * E.L * 3, E.cent + 3 * (int64_t)j, E.L,
+                                   lane);
+    if (lane == 0) E.out[o] = (float)r;
+}
+
+// The queued entries (batch-wide bit indices), one wave each.
+__global__ __launch_bounds__(256) void batch_flag_eval_kernel(BatchArgs b)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t m = min((int64_t)b.counters[kXcd], (int64_t)kFlagListCap);
+    for (int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); k < m; k += (int64_t)gridDim.x * 4) {
+        const int64_t gb = b.flag_list[k];
+        const int64_t gw = gb >> 5;
+        const int e = find_ens(b.word_prefix, b.n_ens, gw);
+        const BatchEns& E = b.ens[e];
+        const int64_t o = (gw - b.word_prefix[e]) * 32 + (gb & 31);
+        if (o < (int64_t)E.n * (E.n - 1) / 2) batch_eval_one(E, o, lane);
+    }
+}
+
+// Bits the queue could not hold: each wave takes 64 consecutive words of the
+// batch-wide bitmap (one per lane) and re-evaluates their set bits one pair
+// at a time.
 __global__ __launch_bounds__(256) void batch_fallback_kernel(BatchArgs b)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t n_waves = (int64_t)gridDim.x * 4;
-    const uint32_t* bits0 = b.ens[0].bits;  // the bitmaps are one array, ensemble 0 first
+    const uint32_t* bits0 = b.ens_bits0;
     for (int64_t w0 = wave * 64; w0 < b.total_words; w0 += n_waves * 64) {
         const int64_t wi = w0 + lane;
         uint32_t word = wi < b.total_words ? bits0[wi] : 0u;
-        uint32_t done = 0;
         while (true) {
             const uint64_t m = __ballot(word != 0u);
             if (!m) break;
@@ -212,17 +242,9 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(BatchArgs b)
             const int e = find_ens(b.word_prefix, b.n_ens, gw);
             const BatchEns& E = b.ens[e];
             const int64_t o = (gw - b.word_prefix[e]) * 32 + bit;
-            const int64_t n_out = (int64_t)E.n * (E.n - 1) / 2;
-            if (o >= n_out) continue;
-            const int i = tri_row(o, E.n);
-            const int j = (int)(o - ((int64_t)i * E.n - (int64_t)i * (i + 1) / 2)) + i + 1;
-            const double r = fallback_pair(E.xyz + (int64_t)i * E.L * 3, E.cent + 3 * (int64_t)i,
-                                           E.xyz + (int64_t)j * E.L * 3, E.cent + 3 * (int64_t)j,
-                                           E.L, lane);
-            if (lane == 0) E.out[o] = (float)r;
-            ++done;
+            if (o >= (int64_t)E.n * (E.n - 1) / 2) continue;
+            batch_eval_one(E, o, lane);
         }
-        if (lane == 0 && done) atomicAdd(b.counters + kXcd, done);
     }
 }
 
@@ -233,7 +255,7 @@ void launch_batch_prep(const BatchArgs& b, hipStream_t s)
     const int cw = (b.total_conf + 3) / 4;
     batch_fit0_kernel<<<cw, 256, 0, s>>>(b);
     batch_mean_partial_kernel<<<b.total_slices, 256, 0, s>>>(b);
-    batch_mean_final_kernel<<<(b.total_coords + 3) / 4, 256, 0, s>>>(b);
+    batch_mean_final_kernel<<<(b.total_coords + 255) / 256, 256, 0, s>>>(b);
     batch_ref_gram_kernel<<<b.n_ens, 256, 0, s>>>(b);
     const int pw = (b.total_pad + 3) / 4;
     if (b.mode == kGramBF16x3)
@@ -256,6 +278,8 @@ void launch_batch_gram(const BatchArgs& b, uint32_t* zero_base, int64_t zero_wor
 
 void launch_batch_fallback(const BatchArgs& b, hipStream_t s)
 {
+    launch_flag_compact(b.ens_bits0, b.total_words, b.flag_list, kFlagListCap, b.counters + kXcd, s);
+    batch_flag_eval_kernel<<<1024, 256, 0, s>>>(b);
     const int64_t waves = (b.total_words + 63) / 64;
     const int64_t blocks = std::min<int64_t>(2048, std::max<int64_t>(1, (waves + 3) / 4));
     batch_fallback_kernel<<<(int)blocks, 256, 0, s>>>(b);

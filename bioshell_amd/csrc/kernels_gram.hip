@@ -57,7 +57,92 @@ __global__ void zero_words_kernel(uint4* p, int64_t n4)
         p[v] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// FP64 re-evaluation of flagged outputs: each wave takes 64 bitmap words (one
+// Queues the set bits of the flag bitmap: one thread per word; a wave scan of
+// the words' bit counts and one atomic per wave give each word its slots.
+// Queued bits are cleared; bits that do not fit in the list (cap entries)
+// stay set for fallback_kernel.  *count receives the number of set bits.
+__global__ __launch_bounds__(256) void flag_compact_kernel(uint32_t* __restrict__ bits,
+                                                           int64_t n_words,
+                                                           int64_t* __restrict__ list, int cap,
+                                                           uint32_t* __restrict__ count)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t base_w = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63); base_w < n_words;
+         base_w += stride) {
+        const int64_t w = base_w + lane;
+        uint32_t word = w < n_words ? bits[w] : 0u;
+        const int p = __popc(word);
+        int incl = p;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        const int total = __shfl(incl, 63);
+        if (total == 0) continue;
+        uint32_t start = 0;
+        if (lane == 63) start = atomicAdd(count, (uint32_t)total);
+        start = __shfl(start, 63);
+        int64_t pos = (int64_t)start + (incl - p);
+        if (p) {
+            uint32_t left = word;
+            while (word) {
+                const int b = __ffs(word) - 1;
+                word &= word - 1u;
+                if (pos < cap) {
+                    list[pos] = w * 32 + b;
+                    left &= ~(1u << b);
+                }
+                ++pos;
+            }
+            bits[w] = left;
+        }
+    }
+}
+
+// Decodes output o of a triangle (n x n, i < j) or of a dense n_rows x out_ld block.
+__device__ inline void decode_output(int64_t o, int triangle, int n, int64_t out_ld, int& i, int& j)
+{
+    if (triangle) {
+        i = tri_row(o, n);
+        j = (int)(o - ((int64_t)i * n - (int64_t)i * (i + 1) / 2)) + i + 1;
+    } else {
+        i = (int)(o / out_ld);
+        j = (int)(o - (int64_t)i * out_ld);
+    }
+}
+
+// FP64 re-evaluation of the queued outputs: one wave per entry, the 64 lanes
+// splitting the atoms.
+__global__ __launch_bounds__(256) void flag_eval_kernel(const int64_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ count, int cap,
+                                                        const double* __restrict__ xa,
+                                                        const double* __restrict__ ca,
+                                                        const double* __restrict__ xb,
+                                                        const double* __restrict__ cb, int L,
+                                                        int triangle, int n, int64_t out_ld,
+                                                        void* __restrict__ out, int out_f64)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t m = min((int64_t)*count, (int64_t)cap);
+    for (int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); k < m; k += (int64_t)gridDim.x * 4) {
+        const int64_t o = list[k];
+        int i, j;
+        decode_output(o, triangle, n, out_ld, i, j);
+        const double r = fallback_pair(xa + (int64_t)i * L * 3, ca + 3 * (int64_t)i,
+                                       xb + (int64_t)j * L * 3, cb + 3 * (int64_t)j, L, lane);
+        if (lane == 0) {
+            if (out_f64)
+                static_cast<double*>(out)[o] = r;
+            else
+                static_cast<float*>(out)[o] = (float)r;
+        }
+    }
+}
+
+// FP64 re-evaluation of flagged outputs left in the bitmap (those the list
+// could not hold): each wave takes 64 bitmap words (one
 // per lane) and processes their set bits one pair at a time, the 64 lanes
 // splitting the atoms; textbook QCP with the near-degenerate polish.
 __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ xa,
@@ -67,7 +152,7 @@ __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict_
                                                        const uint32_t* __restrict__ bits,
                                                        int64_t n_out, int triangle, int n,
                                                        int64_t out_ld, void* __restrict__ out,
-                                                       int out_f64, uint32_t* __restrict__ count)
+                                                       int out_f64)
 {
     const int lane = threadIdx.x & 63;
     const int64_t n_words = (n_out + 31) >> 5;
@@ -76,7 +161,6 @@ __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict_
     for (int64_t w0 = wave * 64; w0 < n_words; w0 += n_waves * 64) {
         const int64_t wi = w0 + lane;
         uint32_t word = wi < n_words ? bits[wi] : 0u;
-        uint32_t done = 0;
         while (true) {
             const uint64_t m = __ballot(word != 0u);
             if (!m) break;
@@ -87,13 +171,7 @@ __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict_
             if (lane == src) word &= word - 1u;  // clear the lowest set bit
             if (o >= n_out) continue;
             int i, j;
-            if (triangle) {
-                i = tri_row(o, n);
-                j = (int)(o - ((int64_t)i * n - (int64_t)i * (i + 1) / 2)) + i + 1;
-            } else {
-                i = (int)(o / out_ld);
-                j = (int)(o - (int64_t)i * out_ld);
-            }
+            decode_output(o, triangle, n, out_ld, i, j);
             const double r = fallback_pair(xa + (int64_t)i * L * 3, ca + 3 * (int64_t)i,
                                            xb + (int64_t)j * L * 3, cb + 3 * (int64_t)j, L, lane);
             if (lane == 0) {
@@ -102,9 +180,7 @@ __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict_
                 else
                     static_cast<float*>(out)[o] = (float)r;
             }
-            ++done;
         }
-        if (count && lane == 0 && done) atomicAdd(count, done);
     }
 }
 
@@ -147,14 +223,24 @@ void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s)
 }
 
 void launch_fallback(const double* xyz_a, const double* cent_a, const double* xyz_b,
-                     const double* cent_b, int L, const uint32_t* bits, int64_t n_out,
-                     int triangle, int n, int64_t out_ld, void* out, int out_f64, uint32_t* count,
-                     hipStream_t s)
+                     const double* cent_b, int L, uint32_t* bits, int64_t n_out, int triangle,
+                     int n, int64_t out_ld, void* out, int out_f64, uint32_t* count, int64_t* list,
+                     int cap, hipStream_t s)
 {
     const int64_t n_words = (n_out + 31) / 32;
+    launch_flag_compact(bits, n_words, list, cap, count, s);
+    flag_eval_kernel<<<1024, 256, 0, s>>>(list, count, cap, xyz_a, cent_a, xyz_b, cent_b, L,
+                                          triangle, n, out_ld, out, out_f64);
     const int64_t blocks = std::min<int64_t>(2048, std::max<int64_t>(1, (n_words + 255) / 256));
     fallback_kernel<<<(int)blocks, 256, 0, s>>>(xyz_a, cent_a, xyz_b, cent_b, L, bits, n_out,
-                                                triangle, n, out_ld, out, out_f64, count);
+                                                triangle, n, out_ld, out, out_f64);
+}
+
+void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,
+                         hipStream_t s)
+{
+    const int64_t blocks = std::min<int64_t>(4096, std::max<int64_t>(1, (n_words + 255) / 256));
+    flag_compact_kernel<<<(int)blocks, 256, 0, s>>>(bits, n_words, list, cap, count);
 }
 
 }  // namespace bs

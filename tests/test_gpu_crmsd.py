@@ -199,3 +199,19 @@ def test_gram_kernel_variants(variant, monkeypatch):
                 err = np.abs(got - want)
                 report(f"variant {variant} {method.name} f64={f64}", err)
                 assert err.max() <= TOL[method]
+
+
+def test_flag_queue_overflow_goes_through_bitmap_scan(gpu_ctx):
+    """Rigid copies of two different chains (1,200 each, 12 atoms): residuals
+    from the common reference are large, so duplicate pairs and, at this small
+    atom count, all others are flagged.  More pairs are flagged than the
+    2^20-entry re-evaluation queue holds, so the rest go through the bitmap
+    scan; every value must still match the oracle."""
+    x = np.concatenate([B.generate_ensemble(1, 1200, 12, sigma=0.0),
+                        B.generate_ensemble(2, 1200, 12, sigma=0.0)])
+    got = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx).upper
+    st = gpu_ctx.last_stats()
+    print("flagged", st["flagged_pairs"], "of", got.size)
+    assert st["flagged_pairs"] > (1 << 20)
+    want = O.all_vs_all(x)
+    assert np.abs(got.astype(np.float64) - want).max() <= TOL[B.Method.BF16X3_GRAM]
