@@ -62,6 +62,8 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         if (const char* e = std::getenv("BIOSHELL_GRAM_VARIANT")) ctx->gsc.variant = std::atoi(e);
         ctx->budget.set_cap(hbm_budget_bytes);
         BS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        BS_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        for (auto& b : ctx->stream_blk) b.bind(&ctx->budget);
         ctx->prep.bind(&ctx->budget);
         ctx->gsc.bind(&ctx->budget);
         ctx->ws_xyz.bind(&ctx->budget);
@@ -84,8 +86,15 @@ void bs_context_destroy(bs_context* ctx)
         ctx->ring.clear();
         if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
         ctx->h_counts = nullptr;
-        ctx->gsc.flags.reset();
-        ctx->gsc.tiles.reset();
+        ctx->gsc.reset();
+        for (auto& b : ctx->stream_blk) b.reset();
+        for (int k = 0; k < 2; ++k) {
+            if (ctx->h_blk[k]) (void)hipHostFree(ctx->h_blk[k]);
+            if (ctx->ev_compute[k]) (void)hipEventDestroy(ctx->ev_compute[k]);
+            if (ctx->ev_copy[k]) (void)hipEventDestroy(ctx->ev_copy[k]);
+        }
+        if (ctx->h_blk_counts) (void)hipHostFree(ctx->h_blk_counts);
+        if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
         ctx->ws_xyz.reset();
         ctx->ws_out.reset();
         ctx->ws_misc.reset();

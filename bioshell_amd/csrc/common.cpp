@@ -1,7 +1,9 @@
 // common.cpp -- error state, device-memory budget, buffers and timers.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <thread>
 
 namespace bs {
 
@@ -102,6 +104,36 @@ float Timer::ms() const
     float v = 0.0f;
     BS_HIP(hipEventElapsedTime(&v, a, b));
     return v;
+}
+
+// Copies a list of (dst, src, bytes) spans with up to 16 threads.
+void parallel_copy(const std::vector<Span>& spans)
+{
+    size_t total = 0;
+    for (const Span& s : spans) total += s.bytes;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<size_t>(std::min(16u, hw), std::max<size_t>(1, total >> 24));
+    if (nt <= 1) {
+        for (const Span& s : spans) std::memcpy(s.dst, s.src, s.bytes);
+        return;
+    }
+    // thread t copies bytes [t total / nt, (t+1) total / nt) of the concatenation
+    auto work = [&](int t) {
+        const size_t lo = total * t / nt, hi = total * (t + 1) / nt;
+        size_t pos = 0;
+        for (const Span& s : spans) {
+            const size_t a = std::max(lo, pos), b = std::min(hi, pos + s.bytes);
+            if (a < b)
+                std::memcpy(static_cast<char*>(s.dst) + (a - pos),
+                            static_cast<const char*>(s.src) + (a - pos), b - a);
+            pos += s.bytes;
+            if (pos >= hi) break;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
 }
 
 }  // namespace bs

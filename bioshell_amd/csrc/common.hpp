@@ -106,6 +106,14 @@ private:
     size_t bytes_ = 0;
 };
 
+// Copies a list of (dst, src, bytes) spans with up to 16 threads.
+struct Span {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+void parallel_copy(const std::vector<Span>& spans);
+
 // Makes `dev` the current device for a scope.
 struct DeviceGuard {
     int prev = -1;

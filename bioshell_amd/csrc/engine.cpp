@@ -101,27 +101,35 @@ bs_run_stats RunRecord::read() const
     st.prep_ms = span(kRecPrep);
     st.main_ms = span(kRecMain);
     st.fallback_ms = span(kRecFallback);
-    st.flagged_pairs = gram && h_count ? *h_count : 0;
+    st.flagged_pairs = gram && h_count ? *h_count : base.flagged_pairs;
     return st;
 }
 
-void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
+PrepView view_of(const Prepared& P, int first, int count)
+{
+    BS_REQUIRE(first >= 0 && first % kTileRows == 0 && first <= P.n, "bad prepared-set view");
+    PrepView v;
+    v.n = count < 0 ? P.n - first : std::min(count, P.n - first);
+    v.panels = static_cast<const char*>(P.panels.get()) +
+               (size_t)(first / kConfBlock) * P.n_kblk * kPanelBlockU4 * 16;
+    v.consts = P.consts.as<double>() + first;
+    v.xyz = P.xyz + (size_t)first * P.L * 3;
+    v.cent = P.cent.as<double>() + (size_t)first * 3;
+    v.ld = P.n_pad;
+    v.L = P.L;
+    v.n_kblk = P.n_kblk;
+    v.mode = P.mode;
+    return v;
+}
+
+void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScratch& sc,
               hipStream_t s, RunRecord* rec)
 {
     BS_REQUIRE(A.L == B.L && A.mode == B.mode, "prepared sets disagree on atom count or mode");
-    const int tiles_r = A.n_pad / kTileRows, tiles_c = B.n_pad / kTileRows;
+    const int tiles_r = (A.n + kTileRows - 1) / kTileRows, tiles_c = (B.n + kTileRows - 1) / kTileRows;
     BS_REQUIRE(tiles_r < 65536 && tiles_c < 65536, "too many conformations for one launch");
-    const uint64_t key = ((uint64_t)tiles_r << 33) | ((uint64_t)tiles_c << 1) | (run.triangle ? 1 : 0);
-    if (key != sc.tiles_key) {
-        const size_t cap = (size_t)tiles_r * tiles_c;
-        std::vector<uint32_t> h(cap);
-        const int nt = build_tile_order(tiles_r, tiles_c, run.triangle, 8, h.data());
-        uint32_t* d = static_cast<uint32_t*>(sc.tiles.ensure((size_t)nt * 4 + 4, "tile order"));
-        BS_HIP(hipMemcpyAsync(d, h.data(), (size_t)nt * 4, hipMemcpyHostToDevice, s));
-        BS_HIP(hipStreamSynchronize(s));  // h goes out of scope
-        sc.tiles_key = key;
-        sc.tiles_n = nt;
-    }
+    int n_tiles = 0;
+    const uint32_t* order = sc.tile_order(tiles_r, tiles_c, run.triangle, s, &n_tiles);
     // Flagged outputs are marked in a bitmap indexed by output position (one
     // bit per pair: 6.25 MB for 50M pairs), so there is no capacity limit and
     // no host round trip; after the Gram kernel the set bits are queued (the
@@ -136,18 +144,18 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     launch_zero_words(flags, (int64_t)n_words + 4, s);
 
     GramArgs g{};
-    g.panels_a = A.panels.get();
-    g.panels_b = B.panels.get();
-    g.consts_a = A.consts.as<double>();
-    g.consts_b = B.consts.as<double>();
-    g.lda = A.n_pad;
-    g.ldb = B.n_pad;
+    g.panels_a = A.panels;
+    g.panels_b = B.panels;
+    g.consts_a = A.consts;
+    g.consts_b = B.consts;
+    g.lda = A.ld;
+    g.ldb = B.ld;
     g.n_rows = A.n;
     g.n_cols = B.n;
     g.L = A.L;
     g.n_kblk = A.n_kblk;
-    g.tiles = sc.tiles.as<uint32_t>();
-    g.n_tiles = sc.tiles_n;
+    g.tiles = order;
+    g.n_tiles = n_tiles;
     g.triangle = run.triangle ? 1 : 0;
     g.out_ld = run.out_ld;
     g.out = run.d_out;
@@ -170,14 +178,64 @@ void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScra
     if (rec) rec->stop(kRecMain, s);
     BS_HIP(hipGetLastError());
     if (rec) rec->start(kRecFallback, s);
-    launch_fallback(A.xyz, A.cent.as<double>(), B.xyz, B.cent.as<double>(), A.L, flags,
+    launch_fallback(A.xyz, A.cent, B.xyz, B.cent, A.L, flags,
                     (int64_t)n_out, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, list,
                     kFlagListCap, s);
     if (rec) rec->stop(kRecFallback, s);
     BS_HIP(hipGetLastError());
     if (rec && rec->h_count)
         BS_HIP(hipMemcpyAsync(rec->h_count, count, 4, hipMemcpyDeviceToHost, s));
+    if (run.h_count) BS_HIP(hipMemcpyAsync(run.h_count, count, 4, hipMemcpyDeviceToHost, s));
     if (rec) rec->gram = true;
+}
+
+void GramScratch::reset()
+{
+    if (order_done) (void)hipEventSynchronize(order_done);
+    flags.reset();
+    for (auto& o : orders) {
+        o.buf.reset();
+        o.key = ~0ull;
+        o.n = 0;
+    }
+    if (h_order) (void)hipHostFree(h_order);
+    h_order = nullptr;
+    h_order_cap = 0;
+    if (order_done) (void)hipEventDestroy(order_done);
+    order_done = nullptr;
+}
+
+const uint32_t* GramScratch::tile_order(int tiles_r, int tiles_c, bool triangle, hipStream_t s,
+                                        int* n)
+{
+    const uint64_t key = ((uint64_t)tiles_r << 33) | ((uint64_t)tiles_c << 1) | (triangle ? 1 : 0);
+    for (auto& o : orders)
+        if (o.key == key) {
+            *n = o.n;
+            return o.buf.as<uint32_t>();
+        }
+    // The pinned staging is reused: wait until its last upload has been read.
+    if (!order_done) BS_HIP(hipEventCreateWithFlags(&order_done, hipEventDisableTiming));
+    BS_HIP(hipEventSynchronize(order_done));
+    const size_t cap = (size_t)tiles_r * tiles_c;
+    if (cap > h_order_cap) {
+        if (h_order) BS_HIP(hipHostFree(h_order));
+        h_order = nullptr;
+        h_order_cap = 0;
+        BS_HIP(hipHostMalloc((void**)&h_order, cap * 4, hipHostMallocDefault));
+        h_order_cap = cap;
+    }
+    TileOrder& o = orders[next_order];
+    next_order = (next_order + 1) % kOrders;
+    o.key = ~0ull;  // the upload is ordered on s after every earlier launch that read the slot
+    const int nt = build_tile_order(tiles_r, tiles_c, triangle, 8, h_order);
+    uint32_t* d = static_cast<uint32_t*>(o.buf.ensure((size_t)nt * 4 + 4, "tile order"));
+    BS_HIP(hipMemcpyAsync(d, h_order, (size_t)nt * 4, hipMemcpyHostToDevice, s));
+    BS_HIP(hipEventRecord(order_done, s));
+    o.key = key;
+    o.n = nt;
+    *n = nt;
+    return d;
 }
 
 void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m, void* d_out,
@@ -205,7 +263,7 @@ void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m
         run.triangle = true;
         run.d_out = d_out;
         run.out_f64 = out_f64;
-        run_gram(ctx->prep, ctx->prep, run, ctx->gsc, s, rec);
+        run_gram(view_of(ctx->prep), view_of(ctx->prep), run, ctx->gsc, s, rec);
         rec->base.main_launches = 1;
         rec->base.tile_m = kTileRows;
         rec->base.tile_n = kTileRows;

@@ -36,6 +36,20 @@ struct Prepared {
     }
 };
 
+// Rows [first, first + count) of a prepared set, as the Gram launch reads
+// them; first is a multiple of 64 (a whole number of 64-row tiles).
+struct PrepView {
+    const void* panels = nullptr;
+    const double* consts = nullptr;
+    const double* xyz = nullptr;
+    const double* cent = nullptr;
+    int ld = 0;         // constant stride (the set's n_pad)
+    int n = 0;          // rows in the view
+    int L = 0, n_kblk = 0;
+    GramMode mode = kGramBF16x3;
+};
+PrepView view_of(const Prepared& P, int first = 0, int count = -1);
+
 // Bytes prepare() will hold for n conformations of L atoms.
 size_t prepared_bytes(int n, int L);
 
@@ -54,6 +68,7 @@ struct GramThresholds {
 GramThresholds gram_thresholds(GramMode mode, int L, int n_kblk);
 
 struct GramRun {
+    uint32_t* h_count = nullptr;  // pinned: receives the flagged-pair count (async)
     bool triangle = true;
     void* d_out = nullptr;
     bool out_f64 = false;
@@ -62,13 +77,31 @@ struct GramRun {
 
 // Scratch a Gram run needs besides the prepared sets.
 struct GramScratch {
-    DevBuf flags, tiles;
-    uint64_t tiles_key = ~0ull;
-    int tiles_n = 0;
+    // Tile orders of the last few launch shapes (tile rows x tile cols x
+    // triangle): a streamed all-vs-all alternates between a few shapes.
+    struct TileOrder {
+        uint64_t key = ~0ull;
+        int n = 0;
+        DevBuf buf;
+    };
+    static constexpr int kOrders = 6;
+    DevBuf flags;
+    TileOrder orders[kOrders];
+    int next_order = 0;
     int max_wg = 0;               // persistent Gram work-groups (2 per CU)
     int skew_sleeps = 0;          // phase offset of the second half of the grid (simple variant)
     int variant = 0;              // 0: one tile per work-group, 1: software-pipelined, persistent
-    void bind(Budget* b) { flags.bind(b); tiles.bind(b); }
+    uint32_t* h_order = nullptr;  // pinned staging of a tile order being uploaded
+    size_t h_order_cap = 0;
+    hipEvent_t order_done = nullptr;
+    void bind(Budget* b)
+    {
+        flags.bind(b);
+        for (auto& o : orders) o.buf.bind(b);
+    }
+    void reset();
+    // Device tile order for the shape; uploaded on first use (stream-ordered).
+    const uint32_t* tile_order(int tiles_r, int tiles_c, bool triangle, hipStream_t s, int* n);
 };
 
 // Events and host-visible results of one call, read lazily.
@@ -89,7 +122,7 @@ enum { kRecTotal = 0, kRecPrep = 1, kRecMain = 2, kRecFallback = 3 };
 // Gram contraction + QCP epilogue + FP64 re-evaluation of flagged pairs over
 // rows of A x cols of B (or the upper triangle of A when triangle).  Asynchronous:
 // the flagged-pair count is copied to rec->h_count when rec is given.
-void run_gram(const Prepared& A, const Prepared& B, const GramRun& run, GramScratch& sc,
+void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScratch& sc,
               hipStream_t s, RunRecord* rec);
 
 // Workspace of one in-flight query batch (query prep, score matrix, top-k).
@@ -121,6 +154,13 @@ struct bs_context {
     bs::Prepared prep;         // the last ensemble's prepared set
     bs::GramScratch gsc;
     bs::DevBuf ws_xyz, ws_out, ws_misc;
+    hipStream_t copy_stream = nullptr;  // device-to-host copies overlapping the compute stream
+    bs::DevBuf stream_blk[2];          // streamed all-vs-all: result blocks on the device
+    void* h_blk[2] = {nullptr, nullptr};  // pinned result blocks
+    size_t h_blk_cap = 0;
+    uint32_t* h_blk_counts = nullptr;  // pinned flagged counts, one per block
+    size_t h_blk_counts_cap = 0;
+    hipEvent_t ev_compute[2] = {}, ev_copy[2] = {};
     bs::QueryWorkspace qws;    // synchronous query calls
     std::vector<bs::RunRecord> ring;  // the last calls' records
     size_t ring_next = 0, ring_used = 0;

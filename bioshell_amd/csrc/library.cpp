@@ -36,9 +36,7 @@ void QueryWorkspace::reset()
     xyz.reset();
     scores.reset();
     topk.reset();
-    gsc.flags.reset();
-    gsc.tiles.reset();
-    gsc.tiles_key = ~0ull;
+    gsc.reset();
 }
 
 // Enqueues the whole query path on stream s: H2D of the queries (from
@@ -64,7 +62,7 @@ void enqueue_query(bs_library* lib, QueryWorkspace& ws, const double* q_host, in
     run.out_ld = n_lib;
     ws.gsc.max_wg = lib->ctx->gsc.max_wg;
     ws.gsc.variant = lib->ctx->gsc.variant;
-    run_gram(ws.prep, lib->P, run, ws.gsc, s, rec);
+    run_gram(view_of(ws.prep), view_of(lib->P), run, ws.gsc, s, rec);
     if (k > 0) {
         uint32_t* idx = static_cast<uint32_t*>(ws.topk.ensure((size_t)n_q * k * 8, "top-k"));
         float* val = reinterpret_cast<float*>(idx + (size_t)n_q * k);

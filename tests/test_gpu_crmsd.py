@@ -215,3 +215,28 @@ def test_flag_queue_overflow_goes_through_bitmap_scan(gpu_ctx):
     assert st["flagged_pairs"] > (1 << 20)
     want = O.all_vs_all(x)
     assert np.abs(got.astype(np.float64) - want).max() <= TOL[B.Method.BF16X3_GRAM]
+
+
+@pytest.mark.parametrize("n,L,tile", [(300, 40, 64), (1000, 77, 256), (130, 20, 4096), (257, 300, 128)])
+def test_streamed_equals_one_shot(gpu_ctx, n, L, tile):
+    """Blocks of tile x tile pairs (diagonal blocks as triangles, edge blocks
+    partial) reassemble into exactly the one-shot triangle."""
+    x = B.generate_ensemble(31 + n, n, L, sigma=(0.5, 4.0), near_fraction=0.05)
+    for m in (B.Method.BF16X3_GRAM, B.Method.FP32_GRAM):
+        one = B.crmsd_all_vs_all(x, m, ctx=gpu_ctx).upper
+        st = B.crmsd_all_vs_all_streamed(x, m, tile=tile, ctx=gpu_ctx).upper
+        assert np.array_equal(one, st), (m, n, tile)
+    nb = (n + min(tile, (n + 63) // 64 * 64) - 1) // min(tile, (n + 63) // 64 * 64)
+    assert gpu_ctx.last_stats()["main_launches"] == nb * (nb + 1) // 2
+    want = O.all_vs_all(x)
+    assert np.abs(st.astype(np.float64) - want).max() <= TOL[B.Method.FP32_GRAM]
+
+
+def test_streamed_plain_and_errors(gpu_ctx):
+    x = B.generate_ensemble(8, 90, 30, sigma=1.0)
+    st = B.crmsd_all_vs_all_streamed(x, B.Method.FP64_PLAIN, tile=64, ctx=gpu_ctx).upper
+    assert np.abs(st.astype(np.float64) - O.all_vs_all(x)).max() <= 1e-5 + 1e-6 * 20
+    with pytest.raises(B.BioshellGpuError):
+        B.crmsd_all_vs_all_streamed(x, tile=100, ctx=gpu_ctx)  # not a multiple of 64
+    with pytest.raises(B.BioshellGpuError):
+        B.crmsd_all_vs_all_streamed(x, tile=0, ctx=gpu_ctx)
