@@ -113,45 +113,56 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
 #undef BS_STASH
 }
 
-// One 64 x 64 tile of pairs by a 4-wave work-group: K loop over atom blocks
-// with the operand blocks double-buffered through LDS, then the epilogue.
+// s_waitcnt with the vector-memory counter limited to N (gfx9 encoding:
+// vmcnt[3:0] | expcnt 7 << 4 | lgkmcnt << 8 | vmcnt[5:4] << 14); the _LDS0
+// form also waits for every LDS access of the wave (lgkmcnt 0).
+#define BS_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(0x0F70 | ((N) & 15) | (((N) >> 4) << 14))
+#define BS_WAIT_VMCNT_LDS0(N) __builtin_amdgcn_s_waitcnt(0x0070 | ((N) & 15) | (((N) >> 4) << 14))
+
+// Direct global -> LDS 16-byte loads: lane l of the wave writes dst + 16 l.
+#define BS_GLOAD_LDS(SRC, DST)                                                                  \
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(SRC),     \
+                                     (void __attribute__((address_space(3)))*)(DST), 16, 0, 0)
+
+// One 64 x 64 tile of pairs by a 4-wave work-group.  K loop over atom
+// blocks with a three-stage LDS ring fed by direct global->LDS loads (no
+// registers held for staging): stage k holds the four operand blocks (A0, A1,
+// B0, B1: 6 KiB each) of atom block k; wave w streams block w.  Step k waits
+// for its own loads of stage k (vmcnt: the 6 loads of stage k+1 may still be
+// in flight) and for its own LDS reads of stage k-1 (lgkmcnt 0), meets the
+// others at a barrier (after which stage k-1 is free), issues atom block k+2
+// into that stage, and runs the 27 (or 72) MFMAs of block k.  Steps are unrolled by three so that every LDS access names its
+// stage statically; the compiler can then see that reads of stage k do not
+// depend on the loads still in flight into the other stages.  Loads past the
+// last block re-read the last block (in bounds, unused); the loop drains
+// them before the epilogue reuses the LDS.
 template <int MODE>
 __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
-                                            uint4 (&lds)[2][4][kPanelBlockU4], int thread)
+                                            uint4 (&lds)[3][4][kPanelBlockU4], int thread)
 {
-    const int tid = thread, lane = tid & 63, wv = tid >> 6;
+    const int lane = thread & 63, wv = thread >> 6;
     const int wr = wv >> 1, wc = wv & 1;
-
     const uint4* __restrict__ pa = static_cast<const uint4*>(a.panels_a);
     const uint4* __restrict__ pb = static_cast<const uint4*>(a.panels_b);
     const int64_t stride_cb = (int64_t)a.n_kblk * kPanelBlockU4;
-    const uint4* srcA0 = pa + (int64_t)(2 * tr) * stride_cb;
-    const uint4* srcB0 = pb + (int64_t)(2 * tc) * stride_cb;
+    const uint4* src = (wv < 2 ? pa + (int64_t)(2 * tr + wv) * stride_cb
+                               : pb + (int64_t)(2 * tc + wv - 2) * stride_cb) + lane;
+    const int n_kblk = a.n_kblk, last = n_kblk - 1;
 
-    // Six 16-byte loads per thread bring one stage (A0, A1, B0, B1 blocks) in.
-    auto src_of = [&](int u) -> const uint4* {
-        const int idx = tid + 256 * u;
-        const int blk = idx / kPanelBlockU4, off = idx - blk * kPanelBlockU4;
-        return (blk < 2 ? srcA0 + blk * stride_cb : srcB0 + (blk - 2) * stride_cb) + off;
-    };
-    const uint4* s0p = src_of(0);
-    const uint4* s1p = src_of(1);
-    const uint4* s2p = src_of(2);
-    const uint4* s3p = src_of(3);
-    const uint4* s4p = src_of(4);
-    const uint4* s5p = src_of(5);
-    uint4 st0, st1, st2, st3, st4, st5;
-#define BS_GLOAD(KB)                                          \
-    {                                                         \
-        const int64_t o_ = (int64_t)(KB) * kPanelBlockU4;     \
-        st0 = s0p[o_]; st1 = s1p[o_]; st2 = s2p[o_];          \
-        st3 = s3p[o_]; st4 = s4p[o_]; st5 = s5p[o_];          \
+#define BS_ISSUE(KB, ST)                                                          \
+    {                                                                             \
+        const uint4* g_ = src + (int64_t)min((KB), last) * kPanelBlockU4;         \
+        _Pragma("unroll") for (int u_ = 0; u_ < 6; ++u_)                          \
+            BS_GLOAD_LDS(g_ + u_ * 64, &lds[ST][wv][u_ * 64]);                    \
     }
-#define BS_LSTORE(ST)                                         \
-    {                                                         \
-        uint4* d_ = &lds[(ST)][0][0] + tid;                   \
-        d_[0] = st0; d_[256] = st1; d_[512] = st2;            \
-        d_[768] = st3; d_[1024] = st4; d_[1280] = st5;        \
+#define BS_STEP(KB, SC, SN)                                                       \
+    {                                                                             \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                    \
+        BS_WAIT_VMCNT_LDS0(6);                                                    \
+        __builtin_amdgcn_s_barrier();                                             \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                    \
+        BS_ISSUE((KB) + 2, SN)                                                    \
+        if ((KB) < n_kblk) mfma_block<MODE>(lds[SC][wr], lds[SC][2 + wc], lane, acc); \
     }
 
     f32x16 acc[9];
@@ -160,26 +171,17 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
 
-    BS_GLOAD(0)
-    BS_LSTORE(0)
-    __syncthreads();
-    for (int kb = 0; kb < a.n_kblk; ++kb) {
-        const int cur = kb & 1;
-        const bool more = kb + 1 < a.n_kblk;
-        if (more) {
-            BS_GLOAD(kb + 1)
-        }
-        const uint4* A = lds[cur][wr];
-        const uint4* B = lds[cur][2 + wc];
-        mfma_block<MODE>(A, B, lane, acc);
-        if (more) {
-            BS_LSTORE(cur ^ 1)
-        }
-        __syncthreads();
+    BS_ISSUE(0, 0)
+    BS_ISSUE(1, 1)
+    for (int kb = 0; kb < n_kblk; kb += 3) {
+        BS_STEP(kb, 0, 2)
+        BS_STEP(kb + 1, 1, 0)
+        BS_STEP(kb + 2, 2, 1)
     }
-
-#undef BS_GLOAD
-#undef BS_LSTORE
+    BS_WAIT_VMCNT(0);  // no load may still write the LDS the epilogue reuses
+    __syncthreads();
+#undef BS_ISSUE
+#undef BS_STEP
 
     epilogue<64>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
 }

@@ -130,8 +130,8 @@ __device__ inline GramArgs ens_args(const BatchArgs& b, const BatchEns& E)
     return a;
 }
 
-constexpr int kBatchLdsBytes = 4 * epilogue_lds_bytes<32>();  // 60 KiB: four 32-tile waves
-static_assert(kBatchLdsBytes >= 2 * 4 * kPanelBlockU4 * 16, "64-tile staging must fit");
+constexpr int kBatchLdsBytes = 3 * 4 * kPanelBlockU4 * 16;  // 72 KiB: the 64-tile stages
+static_assert(kBatchLdsBytes >= 4 * epilogue_lds_bytes<32>(), "four 32-tile waves must fit");
 static_assert(kBatchLdsBytes >= epilogue_lds_bytes<64>(), "64-tile epilogue must fit");
 
 // Persistent: n_wg work-groups (2 per CU), each pulls items from its XCD's
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256, 2) void batch_gram_kernel(BatchArgs b)
             const BatchEns& E = b.ens[(int)(code >> 32)];
             const GramArgs a = ens_args(b, E);
             gram_tile64<MODE>(a, (int)((code >> 16) & 0xFFFFu), (int)(code & 0xFFFFu),
-                              *reinterpret_cast<uint4(*)[2][4][kPanelBlockU4]>(lds), thread);
+                              *reinterpret_cast<uint4(*)[3][4][kPanelBlockU4]>(lds), thread);
             __syncthreads();  // the next item reuses the LDS
         } else {
             const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

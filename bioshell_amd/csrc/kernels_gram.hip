@@ -34,7 +34,7 @@ namespace {
 template <int MODE>
 __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
 {
-    __shared__ uint4 lds[2][4][kPanelBlockU4];  // 2 stages x (A0, A1, B0, B1) x 6 KiB
+    __shared__ uint4 lds[3][4][kPanelBlockU4];  // 3 stages x (A0, A1, B0, B1) x 6 KiB
     const int wg = blockIdx.x;
     const int per = (a.n_tiles + kXcd - 1) / kXcd;
     const int t = (wg % kXcd) * per + (wg / kXcd);

@@ -13,9 +13,9 @@
 // The previous tile's accumulators are kept half in LDS (slots 0..7, then
 // 8..15 once the first half is consumed) and half in registers; its row and
 // column constants sit in a second LDS buffer.  The epilogue slot is branch
-// free: two Newton steps from the start value that provably bounds the root
-// (qcp.hpp), and a pair that has not converged by then is flagged for the
-// FP64 re-evaluation like any other flagged pair.
+// free: one Newton step and the verification that the root reached is the
+// largest one and converged (qcp.hpp); a pair that fails it is flagged for
+// the FP64 re-evaluation like any other flagged pair.
 #include "gram_common.hpp"
 
 #include <algorithm>
@@ -142,24 +142,12 @@ __global__ __launch_bounds__(256, 1) void gram_pipe_kernel(GramArgs a)
         const int il = il0 + 8 * (s_ >> 2) + (s_ & 3);                                       \
         _Pragma("unroll") for (int q = 0; q < kNumConsts; ++q) ci[q] = pconst[il * 12 + q];  \
         const ShiftedPoly sp_ = shifted_poly(E, ci, cjp);                                    \
-        double mu = shifted_start(sp_);                                                      \
-        double pv = poly_at(sp_, mu), dv = dpoly_at(sp_, mu);                                \
-        double stp = newton_div(pv, dv);                                                     \
-        stp = dv > 0.0 ? stp : 0.0;                                                          \
-        mu -= stp;                                                                           \
-        pv = poly_at(sp_, mu);                                                               \
-        dv = dpoly_at(sp_, mu);                                                              \
-        const bool dpok = dv > 0.0;                                                          \
-        stp = newton_div(pv, dv);                                                            \
-        stp = dpok ? stp : 0.0;                                                              \
-        mu -= stp;                                                                           \
-        const double Hp = sp_.H > 0.0 ? sp_.H : 0.0;                                         \
-        const bool conv = dpok & (fabs(stp) <= 1e-13 * fabs(mu) + 1e-15 * Hp);              \
-        const double scl = fabs(4.0 * mu * mu * mu) + fabs(3.0 * sp_.p3 * mu * mu) +         \
-                           fabs(2.0 * sp_.p2 * mu) + fabs(sp_.p1);                           \
-        const bool good = (conv | ((sp_.aa == 0.0) & !(mu > 0.0))) & !(fabs(dv) <= 1e-4 * scl); \
-        const double dd = sp_.H - mu;                                                        \
-        const double r2 = dd > 0.0 ? 2.0 * dd : 0.0;                                         \
+        ShiftedIter st_{shifted_start(sp_), 0.0, 0.0, 0, false, false, false, false};        \
+        shifted_newton(sp_, st_);                                                            \
+        shifted_verify(sp_, st_);                                                            \
+        const ShiftedOut so_ = shifted_finish(sp_, st_);                                     \
+        const bool good = so_.converged;                                                     \
+        const double r2 = so_.rmsd2_L;                                                       \
         const int i_ = ptr * 64 + il;                                                        \
         const bool ok = pjok & (i_ < a.n_rows) & ((a.triangle == 0) | (i_ < pj));            \
         const double sgg = ci[10] * cjp[10];                                                 \

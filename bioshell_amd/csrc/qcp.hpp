@@ -269,54 +269,79 @@ BS_HD double dpoly_at(const ShiftedPoly& s, double m)
     return ((4.0 * m + 3.0 * s.p3) * m + 2.0 * s.p2) * m + s.p1;
 }
 
+// Newton state of one pair.
+struct ShiftedIter {
+    double mu, dp, step;
+    int it;
+    bool conv, stuck, below, restarted;
+};
+
 // Start: near mu = 0 the quartic is close to p1 mu + p0, so -p0/p1 is the
-// root to first order.  It is used only when it provably bounds every real
-// root from above: all Taylor coefficients of the polynomial at that point
-// (P, P', P''/2, P'''/6, 1) are >= 0.  Newton from an upper bound of a
-// real-rooted polynomial converges monotonically to the largest root;
-// otherwise the start is the upper bound H (lambda <= E0).
-// Evaluated without branches (so it can share a basic block with MFMAs):
-// when p1 <= 0 the candidate is inf or NaN and every test below fails.
+// root to first order; if that is not in [0, H) the start is the upper bound
+// H (lambda <= E0).  Whether the iterate reached is the largest root is
+// decided at the end (shifted_verify).
 BS_HD double shifted_start(const ShiftedPoly& s)
 {
     const double Hp = s.H > 0.0 ? s.H : 0.0;
     const double m = -newton_div(s.p0, s.p1);
-    const double t0 = poly_at(s, m);
-    const double t1d = dpoly_at(s, m);
-    const double t2d = (6.0 * m + 3.0 * s.p3) * m + s.p2;
-    const double t3d = 4.0 * m + s.p3;
-    const bool use = (s.p1 > 0.0) & (m >= 0.0) & (m < Hp) & (t0 >= 0.0) & (t1d >= 0.0) &
-                     (t2d >= 0.0) & (t3d >= 0.0);
+    const bool use = (s.p1 > 0.0) & (m >= 0.0) & (m < Hp);
     return use ? m : Hp;
 }
 
-// Newton state of one pair.
-struct ShiftedIter {
-    double mu, dp;
-    int it;
-    bool conv, stuck;
-};
-
-// One Newton step; returns true when converged (or stuck at a stationary point).
-BS_HD bool shifted_step(const ShiftedPoly& s, ShiftedIter& st)
+// One Newton step from st.mu (branch-free: a non-positive slope stops the
+// iteration and marks the pair stuck).
+BS_HD void shifted_newton(const ShiftedPoly& s, ShiftedIter& st)
 {
     const double p = poly_at(s, st.mu);
-    st.dp = dpoly_at(s, st.mu);
+    const double dp = dpoly_at(s, st.mu);
+    st.stuck = st.stuck | !(dp > 0.0);
+    st.step = dp > 0.0 ? newton_div(p, dp) : 0.0;
+    st.mu -= st.step;
     ++st.it;
-    if (!(st.dp > 0.0)) {  // at or below a stationary point: stop, flag
-        st.stuck = true;
-        return true;
-    }
-    const double step = newton_div(p, st.dp);
-    st.mu -= step;
+}
+
+// At the iterate mu reached by the step st.step: converged when the slope and
+// the Taylor coefficients P''/2 and P'''/6 there are positive (then no root of
+// the real-rooted quartic lies above mu, so mu is the largest root) and the
+// error that Newton's quadratic convergence predicts, |P''/(2P')| step^2, is
+// below the tolerance.  Leaves P'(mu) in st.dp.
+BS_HD void shifted_verify(const ShiftedPoly& s, ShiftedIter& st)
+{
+    const double m = st.mu;
+    const double d1 = dpoly_at(s, m);
+    const double t2 = (6.0 * m + 3.0 * s.p3) * m + s.p2;
+    const double t3 = 4.0 * m + s.p3;
     const double Hp = s.H > 0.0 ? s.H : 0.0;
-    st.conv = fabs(step) <= 1e-13 * fabs(st.mu) + 1e-15 * Hp;
-    return st.conv;
+    const double tol = 1e-13 * fabs(m) + 1e-15 * Hp;
+    // |t2 / d1| step^2 <= tol, without a division
+    const bool small = fabs(t2) * (st.step * st.step) <= tol * fabs(d1);
+    st.dp = d1;
+    st.below = !((d1 > 0.0) & (t2 >= 0.0) & (t3 >= 0.0));
+    st.conv = !st.stuck & !st.below & small;
+}
+
+// Iterates after an unverified first step (rare): a root that is not the
+// largest (a negative Taylor coefficient) or a non-positive slope restarts
+// the iteration once from H, which bounds every root from above, so Newton
+// then decreases monotonically to the largest root.
+BS_HD void shifted_continue(const ShiftedPoly& s, ShiftedIter& st, int max_iter)
+{
+    while (!st.conv && st.it < max_iter) {
+        if ((st.below || st.stuck) && !st.restarted) {
+            st.mu = s.H > 0.0 ? s.H : 0.0;
+            st.restarted = true;
+            st.stuck = false;
+        } else if (st.stuck) {
+            break;
+        }
+        shifted_newton(s, st);
+        shifted_verify(s, st);
+    }
 }
 
 BS_HD ShiftedOut shifted_finish(const ShiftedPoly& s, const ShiftedIter& st)
 {
-    bool conv = st.conv && !st.stuck;
+    bool conv = st.conv;
     if (s.aa == 0.0 && !(st.mu > 0.0)) conv = true;  // exactly aligned pair: mu = max(0, lmax(B))
     // near-double root (a small P' at the root): re-evaluate in FP64
     const double m = st.mu;
@@ -332,33 +357,41 @@ BS_HD ShiftedOut shifted_finish(const ShiftedPoly& s, const ShiftedIter& st)
     return o;
 }
 
-// Solves for the largest root of det(mu I - K') for one pair.
+// Solves for the largest root of det(mu I - K') for one pair: one Newton step
+// from the start, then more only while the verification fails.
+BS_HD void shifted_solve(const ShiftedPoly& s, ShiftedIter& st, int max_iter)
+{
+    shifted_newton(s, st);
+    shifted_verify(s, st);
+    shifted_continue(s, st, max_iter);
+}
+
 BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj,
                              int max_iter = 40)
 {
     const ShiftedPoly s = shifted_poly(E, ci, cj);
-    ShiftedIter st{shifted_start(s), 0.0, 0, false, false};
-    while (st.it < max_iter && !shifted_step(s, st)) {
-    }
+    ShiftedIter st{shifted_start(s), 0.0, 0.0, 0, false, false, false, false};
+    shifted_solve(s, st, max_iter);
     return shifted_finish(s, st);
 }
 
-// Two pairs sharing a column, solved in lockstep: coefficient evaluation and
-// the first two Newton steps are straight-line code for both, so the two
-// dependency chains interleave; only unconverged pairs continue alone.
+// Two pairs sharing a column, solved in lockstep: coefficients, the first
+// Newton step and its verification are straight-line code for both, so the
+// two dependency chains interleave; a pair whose verification fails (rare)
+// continues alone.
 BS_HD void qcp_shifted2(const double* E0, const double* ci0, const double* E1, const double* ci1,
                         const double* cj, ShiftedOut& o0, ShiftedOut& o1, int max_iter = 40)
 {
     const ShiftedPoly s0 = shifted_poly(E0, ci0, cj);
     const ShiftedPoly s1 = shifted_poly(E1, ci1, cj);
-    ShiftedIter t0{shifted_start(s0), 0.0, 0, false, false};
-    ShiftedIter t1{shifted_start(s1), 0.0, 0, false, false};
-    bool d0 = shifted_step(s0, t0);
-    bool d1 = shifted_step(s1, t1);
-    if (!d0) d0 = shifted_step(s0, t0);
-    if (!d1) d1 = shifted_step(s1, t1);
-    while (!d0 && t0.it < max_iter) d0 = shifted_step(s0, t0);
-    while (!d1 && t1.it < max_iter) d1 = shifted_step(s1, t1);
+    ShiftedIter t0{shifted_start(s0), 0.0, 0.0, 0, false, false, false, false};
+    ShiftedIter t1{shifted_start(s1), 0.0, 0.0, 0, false, false, false, false};
+    shifted_newton(s0, t0);
+    shifted_newton(s1, t1);
+    shifted_verify(s0, t0);
+    shifted_verify(s1, t1);
+    if (!t0.conv) shifted_continue(s0, t0, max_iter);
+    if (!t1.conv) shifted_continue(s1, t1, max_iter);
     o0 = shifted_finish(s0, t0);
     o1 = shifted_finish(s1, t1);
 }
