@@ -152,9 +152,11 @@ def serving_leg(args, B, log) -> dict:
         nq = int(rng.integers(1, 65))
         s0 = int(rng.integers(0, 256 - nq + 1))
         reqs.append((b, pools[b][s0:s0 + nq]))
-    # warm the server's slots and kernels
+    # warm every slot and kernel: four concurrent requests per bucket
     for b in range(len(buckets)):
-        srv.submit(b, pools[b][:64], 100).wait()
+        warm = [srv.submit(b, pools[b][:64], 100) for _ in range(4)]
+        for t in warm:
+            t.wait()
     tickets = []
     t_start = time.perf_counter()
     due = t_start
@@ -172,6 +174,12 @@ def serving_leg(args, B, log) -> dict:
     wall = time.perf_counter() - t_start
     lat = np.array([t.latency_ms() for t in tickets])
     pairs = sum(q.shape[0] for _, q in reqs) * n_lib
+    # capacity: the same requests submitted at once (queueing dominates latency)
+    t0 = time.perf_counter()
+    burst = [srv.submit(b, q, 100) for b, q in reqs]
+    for t in burst:
+        t.wait()
+    sat = time.perf_counter() - t0
     srv.close()
     for l in libs:
         l.close()
@@ -189,6 +197,8 @@ def serving_leg(args, B, log) -> dict:
         "mean_ms": round(float(lat.mean()), 3),
         "pairs_per_s": round(pairs / wall, 1),
         "latency": "submit to host-visible top-100 (bs_server_latency_ms)",
+        "capacity_requests_per_s": round(R / sat, 1),
+        "capacity_pairs_per_s": round(pairs / sat, 1),
     }
 
 

@@ -132,6 +132,10 @@ struct QueryWorkspace {
     GramScratch gsc;
     void bind(Budget* b);
     void reset();
+    // Allocates up front what a batch of n_q queries of L atoms against n_lib
+    // conformations with top-k needs, so that such batches never reallocate
+    // (a device free synchronises the whole device).
+    void reserve(int n_q, int n_lib, int L, int k);
 };
 
 // Device bytes a query batch of n_q against n_lib conformations of L atoms needs.

@@ -39,6 +39,21 @@ void QueryWorkspace::reset()
     gsc.reset();
 }
 
+void QueryWorkspace::reserve(int n_q, int n_lib, int L, int k)
+{
+    const int n_pad = round_up(n_q, 64), n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
+    prep.cent.ensure((size_t)n_q * 3 * 8, "centroids");
+    prep.rot.ensure((size_t)n_q * 9 * 8, "rotations");
+    prep.ref.ensure((size_t)(3 * L + 6) * 8, "reference");
+    prep.panels.ensure((size_t)n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16, "operand panels");
+    prep.consts.ensure((size_t)kNumConsts * n_pad * 8, "constants");
+    xyz.ensure((size_t)n_q * L * 3 * 8, "query coordinates");
+    scores.ensure((size_t)n_q * n_lib * 4, "query scores");
+    topk.ensure((size_t)n_q * k * 8, "top-k");
+    const uint64_t n_words = (((uint64_t)n_q * n_lib + 31) / 32 + 3) / 4 * 4;
+    gsc.flags.ensure((size_t)(n_words + 4) * 4 + (size_t)kFlagListCap * 8, "flag bitmap and queue");
+}
+
 // Enqueues the whole query path on stream s: H2D of the queries (from
 // q_host, which must stay valid until the stream passes the copy), prep
 // against the library's reference, Gram + epilogue + FP64 re-evaluation into

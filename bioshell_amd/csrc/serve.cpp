@@ -270,11 +270,27 @@ bs_status bs_server_create(bs_context* ctx, bs_library* const* libs, size_t n_li
         srv->libs.assign(libs, libs + n_libs);
         srv->budget = hbm_budget_bytes;
         for (int i = 0; i < max_inflight; ++i) srv->slots.emplace_back();
+        // Every slot is sized up front for a typical batch (up to 64 queries of
+        // the longest library against the largest one, k up to 1024), so that
+        // serving does not allocate, and free, device memory between requests.
+        int max_n = 0, max_L = 0;
+        for (size_t i = 0; i < n_libs; ++i) {
+            max_n = std::max(max_n, libs[i]->P.n);
+            max_L = std::max(max_L, libs[i]->P.L);
+        }
+        const int pre_q = std::min(64, srv->max_batch_queries);
+        const int pre_k = std::min(1024, max_n);
         for (auto& s : srv->slots) {
             BS_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
             BS_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
             s.ws.bind(&ctx->budget);
             s.ws.gsc.max_wg = ctx->gsc.max_wg;
+            if (hbm_budget_bytes == 0 ||
+                query_workspace_bytes(pre_q, max_n, max_L, pre_k) * srv->slots.size() <= hbm_budget_bytes) {
+                s.ws.reserve(pre_q, max_n, max_L, pre_k);
+                ensure_pinned(s.h_q, s.h_q_cap, (size_t)pre_q * max_L * 3);
+                ensure_pinned(s.h_out, s.h_out_cap, (size_t)pre_q * pre_k * 2);
+            }
         }
         bs_server* raw = srv.get();
         srv->worker = std::thread([raw] { raw->run(); });
