@@ -61,6 +61,11 @@ SIGNATURES = {
                                          ctypes.POINTER(c_size), c_size, ctypes.c_int, ctypes.POINTER(c_vp)]),
     "bs_autotune": (c_status, [c_vp, ctypes.c_int, ctypes.c_char_p]),
     "bs_autotune_load": (c_status, [c_vp, ctypes.c_char_p]),
+    "bs_rmsd_row_stats": (c_status, [c_vp, c_fp, c_size, c_fp, c_fp]),
+    "bs_rmsd_row_stats_device": (c_status, [c_vp, c_vp, c_size, c_vp, c_vp]),
+    "bs_rmsd_members_row_max": (c_status, [c_vp, c_fp, c_size, c_u32p, c_size, c_fp]),
+    "bs_hierarchical_clustering": (c_status, [c_vp, c_fp, c_size, ctypes.c_int, c_u32p, c_u32p, c_fp]),
+    "bs_hierarchical_clustering_device": (c_status, [c_vp, c_vp, c_size, ctypes.c_int, c_u32p, c_u32p, c_fp]),
     "bs_library_create": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.POINTER(c_vp)]),
     "bs_library_destroy": (None, [c_vp]),
     "bs_library_size": (c_size, [c_vp]),

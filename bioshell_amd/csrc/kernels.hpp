@@ -158,6 +158,20 @@ void launch_batch_prep(const BatchArgs& b, hipStream_t s);
 void launch_batch_gram(const BatchArgs& b, uint32_t* zero_base, int64_t zero_words, hipStream_t s);
 void launch_batch_fallback(const BatchArgs& b, hipStream_t s);
 
+// ------------------------------------------------------------ clustering
+// Row minimum and maximum over j != i of a packed triangle (distances >= 0).
+void launch_row_stats(const float* up, int n, float* rmin, float* rmax, hipStream_t s);
+// For each of m members, the largest distance to the other members.
+void launch_members_row_max(const float* up, int n, const unsigned* members, int m, float* out,
+                            hipStream_t s);
+// Agglomerative clustering: dense matrix D (n x n), sizes, state {order, step}.
+void launch_cluster_init(const float* up, int n, float* D, unsigned* sizes, int* state,
+                         hipStream_t s);
+// One merge (closest pair over n_part blocks, then the update); records the merge.
+void launch_cluster_step(float* D, int n, unsigned* sizes, int* state, void* part, int n_part,
+                         int rule, float* res, unsigned* mi, unsigned* mj, float* md, hipStream_t s);
+size_t cluster_candidate_bytes();
+
 // Single-pair superposition (rotation + translation) in FP64.
 void launch_superpose(const double* a, const double* b, int L, double* out13, hipStream_t s);
 

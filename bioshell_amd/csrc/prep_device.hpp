@@ -141,7 +141,7 @@ BS_HD void rotation_fast(const double* M, double* R)
     const double a33 = -xx - yy + zz - l;
     // 2x2 minors of rows (0,1) and (2,3) of the symmetric A = K - l I
     const double s0 = a00 * a11 - a01 * a01, s1 = a00 * a12 - a01 * a02, s2 = a00 * a13 - a01 * a03;
-    const double s3 = a01 * a12 - a11 * a02, s4 = a01 * a13 - a11 * a03, s5 = a02 * a13 - a12 * a03;
+    const double s3 = a01 * a12 - a11 * a02, s4 = a01 * a13 - a11 * a03;
     const double t5 = a22 * a33 - a23 * a23, t4 = a12 * a33 - a13 * a23, t3 = a12 * a23 - a13 * a22;
     const double t2 = a02 * a33 - a03 * a23, t1 = a02 * a23 - a03 * a22, t0 = a02 * a13 - a03 * a12;
     // adjugate (symmetric): rows j = 0..3

@@ -178,6 +178,54 @@ bs_status bs_autotune(bs_context* ctx, bs_method method, const char* cache_path)
 /* Loads a previously written autotune cache into the context. */
 bs_status bs_autotune_load(bs_context* ctx, const char* cache_path);
 
+/* ------------------------------------------------------- clustering */
+
+/* Linkage (merging rule) of agglomerative clustering, the closures of
+ * bioshell-clustering/src/hierarchical/strategies/mod.rs: single_link,
+ * complete_link, average_link, median_link, centroid_link, wards_method. */
+typedef enum bs_linkage {
+    BS_SINGLE_LINK = 0,
+    BS_COMPLETE_LINK = 1,
+    BS_AVERAGE_LINK = 2,
+    BS_MEDIAN_LINK = 3,
+    BS_CENTROID_LINK = 4,
+    BS_WARDS_METHOD = 5
+} bs_linkage;
+
+/* Row minimum and maximum over j != i of an n x n distance matrix given as
+ * its packed f32 upper triangle (distances >= 0).  The row minimum against a
+ * cutoff is retrieve_outliers (bioshell-clustering/src/hierarchical/
+ * hierarchical.rs:198-217); the row maximum is what medoid_by_min_max
+ * minimises (hierarchical.rs:105-131).  n = 1 gives f32::MAX / -f32::MAX. */
+bs_status bs_rmsd_row_stats(bs_context* ctx, const float* upper, size_t n, float* row_min,
+                            float* row_max);
+/* The same on device buffers, asynchronous on the context's stream (n >= 2). */
+bs_status bs_rmsd_row_stats_device(bs_context* ctx, const float* d_upper, size_t n,
+                                   float* d_row_min, float* d_row_max);
+/* For each of the m members (indices into the matrix), the largest distance
+ * to the other members: the inner loop of medoid_by_min_max
+ * (hierarchical.rs:105-131) for one cluster. */
+bs_status bs_rmsd_members_row_max(bs_context* ctx, const float* upper, size_t n,
+                                  const uint32_t* members, size_t m, float* row_max);
+
+/* Agglomerative clustering of the n elements of a packed f32 distance
+ * triangle, as hierarchical_clustering (hierarchical.rs:22-80) with the
+ * matrix of clustering_matrix.rs, on the device: merge k joins the clusters
+ * at current matrix indices merge_i[k] < merge_j[k] at distance merge_d[k]
+ * (n - 1 merges).  The merged cluster takes index i and the last active
+ * cluster moves to index j, so the tree is rebuilt by replaying the merges
+ * (the Python mirror does).  Ties, the sizes[j] argument of the merging rule
+ * and the sizes kept in place on a move are those of the reference.  The
+ * dense matrix needs n^2 * 4 bytes of device memory (n <= 65536). */
+bs_status bs_hierarchical_clustering(bs_context* ctx, const float* upper, size_t n,
+                                     bs_linkage linkage, uint32_t* merge_i, uint32_t* merge_j,
+                                     float* merge_d);
+/* The same with the triangle already in HBM (e.g. from
+ * bs_crmsd_all_vs_all_device with f32 output); synchronous. */
+bs_status bs_hierarchical_clustering_device(bs_context* ctx, const float* d_upper, size_t n,
+                                            bs_linkage linkage, uint32_t* merge_i,
+                                            uint32_t* merge_j, float* merge_d);
+
 /* ------------------------------------------------------- libraries */
 
 /* A device-resident library of n_conf conformations of n_atoms each. */

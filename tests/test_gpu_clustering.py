@@ -1,0 +1,125 @@
+"""Device clustering and row reductions against the oracle's restatement of
+BioShell's clustering (bit-identical merge sequences and reductions)."""
+import ctypes
+import os
+import time
+
+import numpy as np
+import pytest
+
+import bioshell_amd as B
+from bioshell_amd import _native
+from bioshell_amd import clustering as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def device_merges(ctx, up, n, linkage):
+    m = max(n - 1, 1)
+    mi = np.zeros(m, np.uint32)
+    mj = np.zeros(m, np.uint32)
+    md = np.zeros(m, np.float32)
+    B.check(_native.lib().bs_hierarchical_clustering(
+        ctx.handle, np.ascontiguousarray(up, np.float32).ctypes.data_as(_native.c_fp), n,
+        C.LINKAGES.index(linkage), mi.ctypes.data_as(_native.c_u32p), mj.ctypes.data_as(_native.c_u32p),
+        md.ctypes.data_as(_native.c_fp)))
+    return mi[:n - 1], mj[:n - 1], md[:n - 1]
+
+
+def random_upper(rng, n, kind):
+    k = n * (n - 1) // 2
+    if kind == "uniform":
+        return rng.uniform(0.1, 9.0, k).astype(np.float32)
+    if kind == "ties":  # few distinct values: ties decide the order
+        return rng.integers(1, 5, k).astype(np.float32)
+    pts = rng.normal(size=(n, 3)) * 4
+    D = np.sqrt(((pts[:, None] - pts[None]) ** 2).sum(-1)).astype(np.float32)
+    return np.array([D[i, j] for i in range(n) for j in range(i + 1, n)], dtype=np.float32)
+
+
+@pytest.mark.parametrize("linkage", C.LINKAGES)
+@pytest.mark.parametrize("n,kind", [(2, "uniform"), (3, "ties"), (17, "uniform"), (64, "ties"),
+                                    (65, "euclid"), (203, "uniform"), (150, "ties")])
+def test_merges_bit_identical_to_oracle(gpu_ctx, linkage, n, kind):
+    rng = np.random.default_rng(n * 7 + len(kind))
+    up = random_upper(rng, n, kind)
+    got = device_merges(gpu_ctx, up, n, linkage)
+    want = O.hclust(up, n, linkage)
+    assert np.array_equal(got[0], want[0].astype(np.uint32)), "merge i"
+    assert np.array_equal(got[1], want[1].astype(np.uint32)), "merge j"
+    assert np.array_equal(got[2].view(np.uint32), want[2].view(np.uint32)), "merge distance bits"
+
+
+@pytest.mark.parametrize("linkage", ["single", "average", "wards"])
+def test_rmsd_matrix_clustering_and_device_entry(gpu_ctx, linkage):
+    """An all-vs-all RMSD matrix: host triangle and HBM triangle give the same
+    merges as the oracle; the tree has every element once."""
+    import torch
+
+    n, L = 300, 60
+    x = B.generate_ensemble(77, n, L, sigma=(0.3, 3.0), near_fraction=0.1)
+    m = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    root = C.hierarchical_clustering(m, linkage=linkage, ctx=gpu_ctx)
+    mi, mj, md = O.hclust(m.upper, n, linkage)
+    ref = C.tree_from_merges(n, mi, mj, md)
+    assert C.retrieve_data_id(root) == C.retrieve_data_id(ref)
+    assert sorted(C.retrieve_data_id(root)) == list(range(n))
+    d_x = torch.from_numpy(x).cuda()
+    d_up = torch.empty(n * (n - 1) // 2, dtype=torch.float32, device="cuda")
+    B.crmsd_all_vs_all_device(d_x.data_ptr(), n, L, d_up.data_ptr(), B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    gpu_ctx.synchronize()
+    root2 = C.hierarchical_clustering(n, linkage=linkage, ctx=gpu_ctx, device_upper=d_up.data_ptr())
+    assert C.retrieve_data_id(root2) == C.retrieve_data_id(root)
+
+
+def test_reference_tests_through_the_device(gpu_ctx):
+    """test_hierarchical.rs cluster_numbers / cluster_letters / d5.tsv"""
+    data = np.array([6.0, 2.0, 2.5, 5.9], dtype=np.float32)
+    d = lambda a, b: float(np.abs(data[a] - data[b]))
+    root = C.hierarchical_clustering(4, d, "single", ctx=gpu_ctx)
+    assert [float(v) for v in C.retrieve_data(root, data)] == [6.0, np.float32(5.9), 2.0, 2.5]
+    C.balance_clustering_tree(root, d)
+    assert [float(v) for v in C.retrieve_data(root, data)] == [6.0, np.float32(5.9), 2.5, 2.0]
+    letters = ['A', 'Z', 'D', 'Y', 'C', 'X', 'B', 'W', 'F', 'V']
+    dl = lambda a, b: float(abs(ord(letters[a]) - ord(letters[b])))
+    root = C.hierarchical_clustering(len(letters), dl, "single", ctx=gpu_ctx)
+    assert C.retrieve_data(root, letters) == ['A', 'B', 'D', 'C', 'F', 'Z', 'Y', 'X', 'W', 'V']
+    C.balance_clustering_tree(root, dl)
+    assert C.retrieve_data(root, letters) == ['A', 'B', 'C', 'D', 'F', 'V', 'W', 'X', 'Y', 'Z']
+
+
+def test_row_stats_outliers_medoid(gpu_ctx):
+    n, L = 700, 45
+    x = B.generate_ensemble(5, n, L, sigma=(0.5, 3.0), near_fraction=0.05)
+    m = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    lo, hi = C.row_stats(m, ctx=gpu_ctx)
+    wlo, whi = O.row_stats(m.upper, n)
+    assert np.array_equal(lo, wlo) and np.array_equal(hi, whi)
+    cut = float(np.median(wlo))
+    assert C.retrieve_outliers(m, cutoff=cut, ctx=gpu_ctx) == [i for i in range(n) if wlo[i] > np.float32(cut)]
+    # medoid of a cluster: members' row maxima on the device, first minimum
+    members = list(range(0, n, 3))
+    got = C.members_row_max(m, members, ctx=gpu_ctx)
+    D = np.zeros((n, n), np.float32)
+    iu = np.triu_indices(n, 1)
+    D[iu] = m.upper
+    D = D + D.T
+    want = np.array([max(D[a, b] for b in members if b != a) for a in members], np.float32)
+    assert np.array_equal(got, want)
+    lo1, hi1 = C.row_stats(B.RmsdMatrix(1, np.zeros(0, np.float32)), ctx=gpu_ctx)
+    assert lo1[0] == np.float32(np.finfo(np.float32).max) and hi1[0] == -np.float32(np.finfo(np.float32).max)
+
+
+def test_clustering_2000_timing(gpu_ctx):
+    n, L = 2000, 100
+    x = B.generate_ensemble(9, n, L, sigma=(0.5, 3.0))
+    m = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    t0 = time.perf_counter()
+    got = device_merges(gpu_ctx, m.upper, n, "average")
+    dt = time.perf_counter() - t0
+    want = O.hclust(m.upper, n, "average")
+    print(f"device clustering of {n}: {dt * 1e3:.1f} ms")
+    assert np.array_equal(got[2].view(np.uint32), want[2].view(np.uint32))
+    assert np.array_equal(got[0], want[0].astype(np.uint32))
