@@ -26,3 +26,10 @@ from .crmsd import (  # noqa: F401
     tri_index,
     version,
 )
+from .ensemble import (  # noqa: F401,E402
+    Ensemble,
+    EnsembleFit,
+    load_ensemble,
+    superpose_ensemble,
+    superpose_ensemble_device,
+)

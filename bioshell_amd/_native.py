@@ -38,6 +38,34 @@ class RunStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class EnsembleInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_models", ctypes.c_size_t),
+        ("n_atoms", ctypes.c_size_t),
+        ("n_structure_atoms", ctypes.c_size_t),
+        ("n_chains", ctypes.c_size_t),
+        ("n_residues", ctypes.c_size_t),
+        ("format", ctypes.c_int),
+    ]
+
+
+class AtomRecord(ctypes.Structure):
+    _fields_ = [
+        ("serial", ctypes.c_int32),
+        ("res_seq", ctypes.c_int32),
+        ("occupancy", ctypes.c_double),
+        ("temp_factor", ctypes.c_double),
+        ("name", ctypes.c_char * 16),
+        ("alt_loc", ctypes.c_char * 8),
+        ("res_name", ctypes.c_char * 16),
+        ("chain_id", ctypes.c_char * 16),
+        ("i_code", ctypes.c_char * 8),
+        ("element", ctypes.c_char * 16),
+        ("has_element", ctypes.c_int32),
+        ("hetero", ctypes.c_int32),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/bioshell_gpu.h
 SIGNATURES = {
     "bs_last_error": (ctypes.c_char_p, []),
@@ -79,6 +107,19 @@ SIGNATURES = {
     "bs_server_wait": (c_status, [c_vp, ctypes.c_uint64]),
     "bs_server_cancel": (c_status, [c_vp, ctypes.c_uint64]),
     "bs_server_latency_ms": (c_status, [c_vp, ctypes.c_uint64, c_dp]),
+    "bs_superpose_ensemble": (c_status, [c_vp, c_dp, c_size, c_size, c_dp, c_size, c_dp, c_dp, c_dp, c_dp]),
+    "bs_superpose_ensemble_device": (c_status, [c_vp, c_vp, c_size, c_size, c_vp, c_size, c_vp, c_vp, c_vp,
+                                                c_vp]),
+    "bs_ensemble_parse": (c_status, [ctypes.c_char_p, c_size, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                     ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "bs_ensemble_load": (c_status, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                    ctypes.POINTER(c_vp)]),
+    "bs_ensemble_destroy": (None, [c_vp]),
+    "bs_ensemble_get_info": (c_status, [c_vp, ctypes.POINTER(EnsembleInfo)]),
+    "bs_ensemble_copy_coords": (c_status, [c_vp, c_dp]),
+    "bs_ensemble_get_atom": (c_status, [c_vp, c_size, ctypes.POINTER(AtomRecord)]),
+    "bs_ensemble_upload": (c_status, [c_vp, c_vp, c_vp]),
+    "bs_ensemble_write_pdb": (c_status, [c_vp, c_dp, c_size, ctypes.c_char_p]),
     "bs_generate_ensemble": (c_status, [ctypes.c_uint64, c_size, c_size, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, c_dp]),
 }

@@ -57,6 +57,8 @@ STATUS = {
     -6: "NOT_READY",
     -7: "CANCELLED",
     -8: "INTERNAL",
+    -9: "PARSE",
+    -10: "IO",
 }
 
 

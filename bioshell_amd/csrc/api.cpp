@@ -189,6 +189,70 @@ bs_status bs_superpose_f64(bs_context* ctx, const double* a_xyz, const double* b
     });
 }
 
+bs_status bs_superpose_ensemble(bs_context* ctx, const double* xyz, size_t n_conf, size_t n_atoms,
+                                const double* ref_xyz, size_t ref_index, double* rot9, double* trans3,
+                                double* rmsd, double* superposed)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        check_shape(n_conf, n_atoms);
+        BS_REQUIRE(n_conf >= 1, "n_conf must be >= 1");
+        BS_REQUIRE(xyz, "null coordinates");
+        BS_REQUIRE(ref_xyz || ref_index < n_conf, "ref_index %zu out of range (%zu conformations)",
+                   ref_index, n_conf);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        hipStream_t s = ctx->stream;
+        const size_t L = n_atoms, cb = n_conf * L * 24;
+        char* in = static_cast<char*>(ctx->ws_xyz.ensure(cb + L * 24, "superposition input"));
+        auto rup = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t o_tr = rup(n_conf * 72), o_rm = o_tr + rup(n_conf * 24), o_sp = o_rm + rup(n_conf * 8);
+        char* ob = static_cast<char*>(ctx->ws_out.ensure(o_sp + (superposed ? cb : 0), "superposition output"));
+        double* misc = static_cast<double*>(
+            ctx->ws_misc.ensure(superpose_scratch_bytes((int)n_conf, (int)L), "superposition scratch"));
+        BS_HIP(hipMemcpyAsync(in, xyz, cb, hipMemcpyHostToDevice, s));
+        const double* d_ref = reinterpret_cast<const double*>(in) + ref_index * 3 * L;
+        if (ref_xyz) {
+            BS_HIP(hipMemcpyAsync(in + cb, ref_xyz, L * 24, hipMemcpyHostToDevice, s));
+            d_ref = reinterpret_cast<const double*>(in + cb);
+        }
+        double* d_rot = reinterpret_cast<double*>(ob);
+        double* d_tr = reinterpret_cast<double*>(ob + o_tr);
+        double* d_rm = reinterpret_cast<double*>(ob + o_rm);
+        double* d_sp = superposed ? reinterpret_cast<double*>(ob + o_sp) : nullptr;
+        launch_superpose_ensemble(reinterpret_cast<const double*>(in), (int)n_conf, (int)L, d_ref, misc,
+                                  d_rot, d_tr, d_rm, d_sp, s);
+        BS_HIP(hipGetLastError());
+        if (rot9) BS_HIP(hipMemcpyAsync(rot9, d_rot, n_conf * 72, hipMemcpyDeviceToHost, s));
+        if (trans3) BS_HIP(hipMemcpyAsync(trans3, d_tr, n_conf * 24, hipMemcpyDeviceToHost, s));
+        if (rmsd) BS_HIP(hipMemcpyAsync(rmsd, d_rm, n_conf * 8, hipMemcpyDeviceToHost, s));
+        if (superposed) BS_HIP(hipMemcpyAsync(superposed, d_sp, cb, hipMemcpyDeviceToHost, s));
+        BS_HIP(hipStreamSynchronize(s));
+    });
+}
+
+bs_status bs_superpose_ensemble_device(bs_context* ctx, const double* d_xyz, size_t n_conf, size_t n_atoms,
+                                       const double* d_ref, size_t ref_index, double* d_rot9,
+                                       double* d_trans3, double* d_rmsd, double* d_superposed)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        check_shape(n_conf, n_atoms);
+        BS_REQUIRE(n_conf >= 1, "n_conf must be >= 1");
+        BS_REQUIRE(d_xyz, "null coordinates");
+        BS_REQUIRE(d_ref || ref_index < n_conf, "ref_index %zu out of range (%zu conformations)", ref_index,
+                   n_conf);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const size_t L = n_atoms;
+        double* misc = static_cast<double*>(
+            ctx->ws_misc.ensure(superpose_scratch_bytes((int)n_conf, (int)L), "superposition scratch"));
+        launch_superpose_ensemble(d_xyz, (int)n_conf, (int)L, d_ref ? d_ref : d_xyz + ref_index * 3 * L, misc,
+                                  d_rot9, d_trans3, d_rmsd, d_superposed, ctx->stream);
+        BS_HIP(hipGetLastError());
+    });
+}
+
 bs_status bs_crmsd_all_vs_all_device(bs_context* ctx, const double* d_xyz, size_t n_conf,
                                      size_t n_atoms, bs_method method, void* d_out_upper,
                                      int out_f64)

@@ -174,5 +174,11 @@ size_t cluster_candidate_bytes();
 
 // Single-pair superposition (rotation + translation) in FP64.
 void launch_superpose(const double* a, const double* b, int L, double* out13, hipStream_t s);
+// Every conformation of xyz (n x L x 3) onto ref (L x 3): rot n x 9, trans
+// n x 3, rmsd n and, when out is not null, the superposed n x L x 3.  scratch
+// holds superpose_scratch_bytes(n, L); any of rot / trans / rmsd may be null.
+size_t superpose_scratch_bytes(int n, int L);
+void launch_superpose_ensemble(const double* xyz, int n, int L, const double* ref, double* scratch,
+                               double* rot, double* trans, double* rmsd, double* out, hipStream_t s);
 
 }  // namespace bs

@@ -46,6 +46,8 @@ typedef int32_t bs_status;
 #define BS_ERR_NOT_READY (-6)
 #define BS_ERR_CANCELLED (-7)
 #define BS_ERR_INTERNAL (-8)
+#define BS_ERR_PARSE (-9) /* malformed structure text (bioshell_pdb::PDBError's parsing variants) */
+#define BS_ERR_IO (-10)   /* a file could not be read or written */
 
 /* How the pairwise cRMSD is evaluated.
  *  BS_FP64_PLAIN   per-pair kernel, FP64: centre, 3x3 covariance, QCP, sqrt.
@@ -119,6 +121,26 @@ bs_status bs_crmsd_pair_f64(bs_context* ctx, const double* a_xyz, const double* 
  * (bioshell-pdb/src/calc/transformation.rs:22-37, applied as at 171-174). */
 bs_status bs_superpose_f64(bs_context* ctx, const double* a_xyz, const double* b_xyz,
                            size_t n_atoms, double* rot9, double* trans3, double* rmsd);
+
+/* Superposition of every conformation of an ensemble (n_conf x n_atoms x 3)
+ * onto one reference -- ref_xyz (n_atoms x 3), or conformation ref_index when
+ * ref_xyz is NULL: for conformation m, R_m (row-major 3x3, rot9[9m..]) and
+ * t_m (trans3[3m..]) with ref ~= R_m x + t_m, the fit's RMSD (rmsd[m]) and the
+ * superposed coordinates R_m x + t_m (superposed, n_conf x n_atoms x 3).
+ * Each output may be NULL.  FP64 throughout.  This is the Rototranslation of
+ * bioshell-pdb/src/calc/transformation.rs:22-37 (x' = R x + t, apply_mut at
+ * 171-174) fitted per model, the companion of writing superposed models
+ * (bs_ensemble_write_pdb). */
+bs_status bs_superpose_ensemble(bs_context* ctx, const double* xyz, size_t n_conf, size_t n_atoms,
+                                const double* ref_xyz, size_t ref_index, double* rot9,
+                                double* trans3, double* rmsd, double* superposed);
+/* The same on device buffers, asynchronous on the context's stream; d_ref may
+ * be NULL (conformation ref_index is the reference); d_superposed must not
+ * overlap d_xyz. */
+bs_status bs_superpose_ensemble_device(bs_context* ctx, const double* d_xyz, size_t n_conf,
+                                       size_t n_atoms, const double* d_ref, size_t ref_index,
+                                       double* d_rot9, double* d_trans3, double* d_rmsd,
+                                       double* d_superposed);
 
 /* ------------------------------------------------------- all-vs-all */
 
@@ -225,6 +247,84 @@ bs_status bs_hierarchical_clustering(bs_context* ctx, const float* upper, size_t
 bs_status bs_hierarchical_clustering_device(bs_context* ctx, const float* d_upper, size_t n,
                                             bs_linkage linkage, uint32_t* merge_i,
                                             uint32_t* merge_j, float* merge_d);
+
+/* ------------------------------------------------------- ensembles from files */
+
+/* A conformational ensemble read from a multi-model structure file: the atom
+ * records of the first model and the coordinates of the selected atoms in
+ * every model (n_models x n_atoms x 3, f64, the layout every cRMSD entry point
+ * takes).  It is what the reference gives for Deposit::from_file
+ * (bioshell-pdb/src/deposit.rs:81-85) followed, for every model, by
+ * Structure::set_model (structure.rs:277-285) and an atom predicate over
+ * Structure::atoms(): models are numbered in order of appearance, atoms come
+ * from the first model, and a model whose atom count differs from the first
+ * model's is an error (PDBError::WrongAtomsNumberInModel). */
+typedef struct bs_ensemble bs_ensemble;
+
+typedef enum bs_format {
+    BS_FORMAT_AUTO = 0, /* is_cif_file, then is_pdb_file (load_cif.rs:155-167, load_pdb.rs:225-237) */
+    BS_FORMAT_CIF = 1,  /* Deposit::from_cif_reader + PdbAtom::from_cif_data (pdb_atom.rs:130-203) */
+    BS_FORMAT_PDB = 2   /* Deposit::from_pdb_reader (load_pdb.rs:43-115) */
+} bs_format;
+
+/* Atom predicates of bioshell-pdb/src/pdb_atom_filters.rs. */
+typedef enum bs_selection {
+    BS_SELECT_ALL = 0,
+    BS_SELECT_CA = 1,          /* IsCA (pdb_atom_filters.rs:388-394) */
+    BS_SELECT_CB = 2,          /* IsCB (pdb_atom_filters.rs:415-422) */
+    BS_SELECT_NOT_HYDROGEN = 3 /* !is_hydrogen (pdb_atom.rs:352-365) */
+} bs_selection;
+
+typedef struct bs_ensemble_info {
+    size_t n_models;          /* Structure::count_models (structure.rs:255) */
+    size_t n_atoms;           /* selected atoms per model */
+    size_t n_structure_atoms; /* Structure::count_atoms of the first model */
+    size_t n_chains;          /* Structure::count_chains (structure.rs:250-252) */
+    size_t n_residues;        /* Structure::count_residues (structure.rs:236-239) */
+    int format;               /* BS_FORMAT_CIF or BS_FORMAT_PDB */
+} bs_ensemble_info;
+
+/* One atom record (bioshell_pdb::PdbAtom); strings are UTF-8, NUL-terminated
+ * and cut at the field size. */
+typedef struct bs_atom_record {
+    int32_t serial;
+    int32_t res_seq;
+    double occupancy;
+    double temp_factor;
+    char name[16];
+    char alt_loc[8];
+    char res_name[16];
+    char chain_id[16];
+    char i_code[8];
+    char element[16];
+    int32_t has_element;
+    int32_t hetero;
+} bs_atom_record;
+
+/* Reads an ensemble from mmCIF or PDB text (len bytes, not NUL-terminated).
+ * chain_id (or NULL) adds the ByChain predicate (pdb_atom_filters.rs:191-199).
+ * n_threads <= 0 picks up to 16.  Parse errors return BS_ERR_PARSE. */
+bs_status bs_ensemble_parse(const char* text, size_t len, int format, int selection,
+                            const char* chain_id, int n_threads, bs_ensemble** out);
+/* The same from a file; a name ending in .gz is inflated
+ * (bioshell-core/src/io/utils.rs:312-330). */
+bs_status bs_ensemble_load(const char* path, int format, int selection, const char* chain_id,
+                           int n_threads, bs_ensemble** out);
+void bs_ensemble_destroy(bs_ensemble* ens);
+bs_status bs_ensemble_get_info(const bs_ensemble* ens, bs_ensemble_info* info);
+/* Copies the n_models x n_atoms x 3 coordinates out. */
+bs_status bs_ensemble_copy_coords(const bs_ensemble* ens, double* xyz);
+bs_status bs_ensemble_get_atom(const bs_ensemble* ens, size_t k, bs_atom_record* out);
+/* Copies the coordinates to device memory (d_xyz, n_models x n_atoms x 3 f64)
+ * on the context's stream and waits for the copy. */
+bs_status bs_ensemble_upload(bs_context* ctx, const bs_ensemble* ens, double* d_xyz);
+/* Writes models in PDB format, each with write_pdb's records
+ * (bioshell-pdb/src/structure.rs:828-850, lines as impl Display for PdbAtom,
+ * pdb_atom.rs:256-278); more than one model is framed by MODEL / ENDMDL
+ * records, which Deposit::from_pdb_reader reads back as models.  xyz (or NULL
+ * for the ensemble's own coordinates) holds n_models x n_atoms x 3. */
+bs_status bs_ensemble_write_pdb(const bs_ensemble* ens, const double* xyz, size_t n_models,
+                                const char* path);
 
 /* ------------------------------------------------------- libraries */
 

@@ -7,11 +7,16 @@ Run in the development container (it reads the reference's test data from
 
 Fixtures (all data: inputs plus expected outputs):
 
-* ``2jqb_ensembles.npz`` -- the 20 NMR models of 2jqb
-  (bioshell-pdb/tests/test_files/2jqb.cif; the reference's multi-model test
-  at bioshell-pdb/tests/test_structure.rs:156-166 loads the same file), as
-  CA-only (20 x n_ca x 3) and all-ATOM (20 x n_atom x 3) coordinate arrays,
-  with their all-vs-all cRMSD (packed upper triangle, f64).
+* ``structures/*.gz`` -- the reference's structure test files
+  (bioshell-pdb/tests/test_files/), gzip-compressed data: multi-model 2jqb,
+  2gb1 as both mmCIF and PDB, and files with chains, alternate locations,
+  insertion codes, quoted atom names and ions.
+* ``2jqb_ensembles.npz`` -- the 20 NMR models of 2jqb (the reference's
+  multi-model test at bioshell-pdb/tests/test_structure.rs:156-166 loads the
+  same file, 255 atoms per model), read as the reference reads them (first
+  model's atoms, every model's coordinates, ATOM and HETATM records) with the
+  IsCA predicate (20 x 15 x 3) and with all atoms (20 x 255 x 3), with their
+  all-vs-all cRMSD (packed upper triangle, f64).
 * ``config0.npz`` -- all-vs-all cRMSD of the synthetic 200 x 76 ensemble
   (seed 0, sigma 1.5 A) and a SHA-256 of the generated coordinates.
 * ``d5.tsv`` -- copied verbatim from bioshell-clustering/tests/test_files/d5.tsv
@@ -22,6 +27,7 @@ sign(det M) s3), computed independently of the project's QCP oracle.
 """
 from __future__ import annotations
 
+import gzip
 import hashlib
 import os
 import shutil
@@ -52,40 +58,23 @@ def kabsch_all_vs_all(x: np.ndarray) -> np.ndarray:
     return out
 
 
-def read_2jqb(path: str):
-    """CA and all-ATOM coordinates per model from the mmCIF _atom_site loop."""
-    cols, rows = [], []
-    in_loop = False
-    with open(path) as f:
-        for line in f:
-            if line.startswith("_atom_site."):
-                cols.append(line.strip().split(".", 1)[1])
-                in_loop = True
-                continue
-            if in_loop and (line.startswith("ATOM") or line.startswith("HETATM")):
-                rows.append(line.split())
-            elif in_loop and rows and line.startswith("#"):
-                break
-    ix = {c: k for k, c in enumerate(cols)}
-    models: dict[int, dict[str, list]] = {}
-    for r in rows:
-        if r[ix["group_PDB"]] != "ATOM":
-            continue
-        m = int(r[ix["pdbx_PDB_model_num"]])
-        xyz = [float(r[ix["Cartn_x"]]), float(r[ix["Cartn_y"]]), float(r[ix["Cartn_z"]])]
-        d = models.setdefault(m, {"ca": [], "all": []})
-        d["all"].append(xyz)
-        if r[ix["label_atom_id"]] == "CA":
-            d["ca"].append(xyz)
-    keys = sorted(models)
-    ca = np.array([models[k]["ca"] for k in keys])
-    allat = np.array([models[k]["all"] for k in keys])
-    return ca, allat
+STRUCTURES = ("2jqb.cif", "2gb1.cif", "2gb1.pdb", "2fdo.cif", "1c5n.cif", "5edw.cif", "6ins.cif",
+              "1akd_fragm.pdb", "alphaHelix.pdb")
 
 
 def main() -> None:
-    src = os.path.join(REF, "bioshell-pdb/tests/test_files/2jqb.cif")
-    ca, allat = read_2jqb(src)
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from oracle import ingest_oracle as I
+
+    sdir = os.path.join(HERE, "structures")
+    os.makedirs(sdir, exist_ok=True)
+    for name in STRUCTURES:
+        src = os.path.join(REF, "bioshell-pdb/tests/test_files", name)
+        with open(src, "rb") as f, gzip.GzipFile(os.path.join(sdir, name + ".gz"), "wb", mtime=0) as g:
+            g.write(f.read())
+    text = I.read_text(os.path.join(sdir, "2jqb.cif.gz"))
+    ca = I.ensemble(text, selection="ca").xyz
+    allat = I.ensemble(text, selection="all").xyz
     np.savez_compressed(
         os.path.join(HERE, "2jqb_ensembles.npz"),
         ca=ca, ca_rmsd=kabsch_all_vs_all(ca),
@@ -93,7 +82,6 @@ def main() -> None:
     )
     print("2jqb:", ca.shape, allat.shape)
 
-    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
     import bioshell_amd as B  # the generator is host code; no GPU needed
 
     x = B.generate_ensemble(0, 200, 76, sigma=1.5)
