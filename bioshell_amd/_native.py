@@ -120,6 +120,8 @@ SIGNATURES = {
     "bs_ensemble_get_atom": (c_status, [c_vp, c_size, ctypes.POINTER(AtomRecord)]),
     "bs_ensemble_upload": (c_status, [c_vp, c_vp, c_vp]),
     "bs_ensemble_write_pdb": (c_status, [c_vp, c_dp, c_size, ctypes.c_char_p]),
+    "bs_write_distance_tsv": (c_status, [c_fp, c_size, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_char_p]),
     "bs_generate_ensemble": (c_status, [ctypes.c_uint64, c_size, c_size, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, c_dp]),
 }

@@ -13,6 +13,7 @@ functions on top of it, with the same names and results.
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass, field
 from typing import Callable, Sequence
 
@@ -297,6 +298,59 @@ def retrieve_outliers(data, distance: Callable[[int, int], float] | None = None,
         if m > cutoff:
             out.append(i)
     return out
+
+
+def write_tsv(matrix: RmsdMatrix, path: str, labels: Sequence[str] | None = None, decimals: int = -1,
+              both_triangles: bool = False, n_threads: int = 0) -> None:
+    """Writes the matrix as the TSV that the reference's ``clust`` tool reads
+    (bs_write_distance_tsv): ``label_i<TAB>label_j<TAB>value`` for every pair
+    i < j in row order (every ordered pair with ``both_triangles``).  Values
+    are the shortest text that reads back to the same f32 unless
+    ``decimals`` >= 0 is given."""
+    n, up = _upper_of(matrix)
+    arr = None
+    if labels is not None:
+        if len(labels) != n:
+            raise ValueError(f"{len(labels)} labels for {n} elements")
+        enc = [str(l).encode() for l in labels]
+        if any(b"\t" in e or b"\n" in e for e in enc):
+            raise ValueError("labels may not contain tabs or newlines")
+        arr = (ctypes.c_char_p * n)(*enc)
+    check(_native.lib().bs_write_distance_tsv(up.ctypes.data_as(_native.c_fp), n, arr, int(decimals),
+                                               int(bool(both_triangles)), int(n_threads), os.fsencode(path)))
+
+
+def read_tsv(path: str, symmetric: bool = True, default_distance: float = 100.0) -> tuple[RmsdMatrix, list[str]]:
+    """Reads ``label_i label_j value`` lines into a matrix and its labels,
+    numbered in order of first appearance, as ``clust`` loads its input
+    (clust.rs:102-107 with DataMatrixBuilder, symmetric as with --symmetric;
+    pairs absent from the file take ``default_distance``; distances are used
+    as f32, distance_matrix.rs:71-74)."""
+    ids: dict[str, int] = {}
+    vals: dict[tuple[int, int], float] = {}
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if not parts:
+                continue
+            if len(parts) < 3:
+                raise ValueError(f"expected 'label label value': {line!r}")
+            a, b, v = parts[0], parts[1], float(parts[2])
+            for x in (a, b):
+                ids.setdefault(x, len(ids))
+            vals[(ids[a], ids[b])] = v
+            if symmetric:
+                vals[(ids[b], ids[a])] = v
+    n = len(ids)
+    up = np.empty(n * (n - 1) // 2, dtype=np.float32)
+    k = 0
+    for i in range(n):
+        for j in range(i + 1, n):
+            # hierarchical_clustering asks for d(j, i) with j > i (clustering_matrix.rs:11-21)
+            up[k] = np.float32(vals.get((j, i), vals.get((i, j), default_distance)) if symmetric
+                               else vals.get((j, i), default_distance))
+            k += 1
+    return RmsdMatrix(n, up), list(ids)
 
 
 def show_clustering_tree(root: ClusteringTreeNode) -> str:

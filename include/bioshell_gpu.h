@@ -248,6 +248,19 @@ bs_status bs_hierarchical_clustering_device(bs_context* ctx, const float* d_uppe
                                             bs_linkage linkage, uint32_t* merge_i,
                                             uint32_t* merge_j, float* merge_d);
 
+/* Writes a distance matrix (packed f32 upper triangle of n elements) as the
+ * tab-separated "label_i<TAB>label_j<TAB>value" lines that BioShell's clust
+ * tool reads (bioshell-clustering/src/bin/clust.rs:102-107; the layout of its
+ * fixtures tests/test_files/cities.tsv and d5.tsv): one line per pair i < j in
+ * row order, or, with both_triangles != 0, per ordered pair i != j (for clust
+ * runs without --symmetric).  labels holds n NUL-terminated labels, or NULL
+ * for "0".."n-1".  decimals < 0 writes each value in the shortest positional
+ * form that reads back to the same f32 (Rust's `{}` for f32); 0..30 writes that
+ * many decimals of the exact f32 value.  n_threads <= 0 picks up to 16. */
+bs_status bs_write_distance_tsv(const float* upper, size_t n, const char* const* labels,
+                                int decimals, int both_triangles, int n_threads,
+                                const char* path);
+
 /* ------------------------------------------------------- ensembles from files */
 
 /* A conformational ensemble read from a multi-model structure file: the atom

@@ -19,8 +19,9 @@ Fixtures (all data: inputs plus expected outputs):
   all-vs-all cRMSD (packed upper triangle, f64).
 * ``config0.npz`` -- all-vs-all cRMSD of the synthetic 200 x 76 ensemble
   (seed 0, sigma 1.5 A) and a SHA-256 of the generated coordinates.
-* ``d5.tsv`` -- copied verbatim from bioshell-clustering/tests/test_files/d5.tsv
-  (a distance-matrix fixture the reference's clustering tests read).
+* ``d5.tsv``, ``cities.tsv`` -- copied verbatim from
+  bioshell-clustering/tests/test_files/ (the distance-matrix fixtures the
+  reference's clustering tests and DataMatrixDistance doc-test read).
 
 Expected cRMSD values come from numpy's SVD (Kabsch: lambda = s1 + s2 +
 sign(det M) s3), computed independently of the project's QCP oracle.
@@ -93,8 +94,8 @@ def main() -> None:
     )
     print("config0:", x.shape)
 
-    shutil.copyfile(os.path.join(REF, "bioshell-clustering/tests/test_files/d5.tsv"),
-                    os.path.join(HERE, "d5.tsv"))
+    for name in ("d5.tsv", "cities.tsv"):
+        shutil.copyfile(os.path.join(REF, "bioshell-clustering/tests/test_files", name), os.path.join(HERE, name))
 
 
 if __name__ == "__main__":
