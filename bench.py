@@ -202,6 +202,206 @@ def serving_leg(args, B, log) -> dict:
     }
 
 
+def synthetic_cif(n_models: int) -> bytes:
+    """A multi-model mmCIF text: the 255 atoms of the first 2jqb model (the
+    reference's multi-model fixture) repeated as n_models models, each with
+    its coordinates displaced."""
+    from oracle import ingest_oracle as I
+
+    raw = I.read_text(os.path.join(ROOT, "tests", "golden", "structures", "2jqb.cif.gz")).decode()
+    head, rest = raw.split("_atom_site.pdbx_PDB_model_num", 1)
+    body, tail = rest.split("\n#", 1)
+    first = [l.split() for l in body.split("\n") if l.strip() and l.split()[-1] == "1"]
+    rng = np.random.default_rng(1)
+    out = [head, "_atom_site.pdbx_PDB_model_num\n"]
+    for m in range(n_models):
+        d = rng.normal(size=(len(first), 3)) * 0.5
+        for t, dd in zip(first, d):
+            t = list(t)
+            t[-1] = str(m + 1)
+            for c in range(3):
+                t[10 + c] = f"{float(t[10 + c]) + dd[c]:.3f}"
+            out.append(" ".join(t) + "\n")
+    out.append("#" + tail)
+    return "".join(out).encode()
+
+
+def next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x) -> dict:
+    """The path's neighbours, each on its own workload: clustering and row
+    reductions over the headline triangle already in HBM, ensemble superposition
+    of the headline ensemble, ensemble ingestion from mmCIF text, and the clust
+    TSV writer.  CPU figures are the oracle (or single-threaded native code where
+    the oracle has no counterpart) on a bounded sample, as stated per entry."""
+    from bioshell_amd import clustering as C
+    from oracle import ingest_oracle as I
+    from oracle import oracle as O
+
+    n, L = x.shape[:2]
+    out = {}
+    dev = d_out.device
+    # ---- row minima / maxima (outliers, medoid) and clustering, triangle in HBM
+    d_lo = torch.empty(n, dtype=torch.float32, device=dev)
+    d_hi = torch.empty(n, dtype=torch.float32, device=dev)
+    import ctypes
+
+    from bioshell_amd import _native
+
+    vp = ctypes.c_void_p
+    B.check(_native.lib().bs_rmsd_row_stats_device(ctx.handle, vp(d_out.data_ptr()), n, vp(d_lo.data_ptr()),
+                                                    vp(d_hi.data_ptr())))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        B.check(_native.lib().bs_rmsd_row_stats_device(ctx.handle, vp(d_out.data_ptr()), n, vp(d_lo.data_ptr()),
+                                                        vp(d_hi.data_ptr())))
+    ctx.synchronize()
+    rs_ms = (time.perf_counter() - t0) * 1e3 / reps
+    tri_bytes = n * (n - 1) // 2 * 4
+    cn = min(n, args.cluster_n)
+    d_tri = d_out
+    if cn < n:  # the packed triangle of the leading cn x cn block
+        ii = np.concatenate([i * n - i * (i + 1) // 2 + (np.arange(i + 1, cn) - i - 1) for i in range(cn)])
+        d_tri = d_out[torch.from_numpy(ii).to(dev)].contiguous()
+        torch.cuda.synchronize()
+    m_i = np.zeros(cn, np.uint32)
+    m_j = np.zeros(cn, np.uint32)
+    m_d = np.zeros(cn, np.float32)
+    t0 = time.perf_counter()
+    B.check(_native.lib().bs_hierarchical_clustering_device(ctx.handle, vp(d_tri.data_ptr()), cn,
+                                                             C.LINKAGES.index("average"),
+                                                             m_i.ctypes.data_as(_native.c_u32p),
+                                                             m_j.ctypes.data_as(_native.c_u32p),
+                                                             m_d.ctypes.data_as(_native.c_fp)))
+    cl_ms = (time.perf_counter() - t0) * 1e3
+    scan_bytes = sum(k * (k - 1) // 2 * 4 for k in range(2, cn + 1))
+    # like for like at a CPU-sized sample: the first 1000 elements
+    sn = 1000
+    sub = d_out.cpu().numpy()
+    idx = np.concatenate([i * n - i * (i + 1) // 2 + (np.arange(i + 1, sn) - i - 1) for i in range(sn)])
+    up_s = np.ascontiguousarray(sub[idx])
+    t0 = time.perf_counter()
+    gi = C.hierarchical_clustering(B.RmsdMatrix(sn, up_s), linkage="average", ctx=ctx)
+    g_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    oi, oj, od = O.hclust(up_s, sn, "average")
+    c_ms = (time.perf_counter() - t0) * 1e3
+    same = C.retrieve_data_id(gi) == C.retrieve_data_id(C.tree_from_merges(sn, oi, oj, od))
+    log(f"clustering {cn}: {cl_ms:.1f} ms; row stats {rs_ms:.3f} ms; sample {sn}: gpu {g_ms:.1f} ms, cpu {c_ms:.1f} ms")
+    out["clustering"] = {
+        "workload": f"average-linkage agglomerative clustering of the {cn} x {cn} f32 triangle in HBM "
+                    "(bs_hierarchical_clustering_device, merge order of the reference)",
+        "n": cn,
+        "ms": round(cl_ms, 2),
+        "roofline": {"bound": "hbm", "achieved": round(scan_bytes / (cl_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(scan_bytes / (cl_ms * 1e-3) / 8e12, 4),
+                     "traffic": None, "algorithmic_bytes": scan_bytes,
+                     "note": "closest-pair scans of the active triangle over all merges, per call"},
+        "sample_1000": {"gpu_ms": round(g_ms, 2), "cpu_ms": round(c_ms, 2), "same_tree": bool(same)},
+        "cpu_baseline": {"value": round(c_ms, 2), "unit": "ms", "cores": 1, "kind": "port",
+                         "sample": "oracle or_hclust (C restatement of hierarchical.rs:22-80) on the first "
+                                   "1000 elements of the same matrix"},
+        "row_stats_ms": round(rs_ms, 4),
+        "row_stats_roofline": {"bound": "hbm", "achieved": round(tri_bytes / (rs_ms * 1e-3) / 1e9, 1),
+                               "peak": 8000.0, "unit": "GB/s",
+                               "frac": round(tri_bytes / (rs_ms * 1e-3) / 8e12, 4), "traffic": None},
+    }
+    del d_lo, d_hi
+    # ---- superposition of every conformation onto conformation 0 (FP64)
+    d_rot = torch.empty((n, 9), dtype=torch.float64, device=dev)
+    d_tr = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    d_rm = torch.empty(n, dtype=torch.float64, device=dev)
+    d_sp = torch.empty_like(d_xyz)
+
+    def sup():
+        B.superpose_ensemble_device(d_xyz.data_ptr(), n, L, None, 0, d_rot.data_ptr(), d_tr.data_ptr(),
+                                    d_rm.data_ptr(), d_sp.data_ptr(), ctx=ctx)
+
+    sup()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sup()
+    ctx.synchronize()
+    sp_ms = (time.perf_counter() - t0) * 1e3 / reps
+    algo = n * L * 24 * 2  # coordinates read once, superposed coordinates written once
+    rm = d_rm.cpu().numpy()
+    k = 300
+    t0 = time.perf_counter()
+    want = np.array([O.superpose(x[0], x[m])[2] for m in range(k)])
+    c_s = time.perf_counter() - t0
+    out["superposition"] = {
+        "workload": f"{n} x {L} ensemble fitted onto conformation 0: R, t, RMSD and the superposed "
+                    "coordinates, FP64, in HBM (bs_superpose_ensemble_device)",
+        "ms": round(sp_ms, 4),
+        "value": round(n / (sp_ms * 1e-3), 1),
+        "unit": "conformations/s",
+        "roofline": {"bound": "hbm", "achieved": round(algo / (sp_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(algo / (sp_ms * 1e-3) / 8e12, 4), "traffic": None},
+        "max_abs_rmsd_diff_vs_oracle": float(np.abs(rm[:k] - want).max()),
+        "cpu_baseline": {"value": round(k / c_s, 1), "unit": "conformations/s", "cores": 1, "kind": "port",
+                         "sample": f"oracle or_superpose on conformations 0..{k - 1}"},
+    }
+    del d_rot, d_tr, d_rm, d_sp
+    log(f"superposition: {sp_ms:.3f} ms")
+    # ---- ingestion: multi-model mmCIF -> CA ensemble
+    text = synthetic_cif(args.ingest_models)
+    B.Ensemble.from_text(text, select="ca", n_threads=16)  # warm the page cache and allocator
+    t0 = time.perf_counter()
+    e = B.Ensemble.from_text(text, select="ca", n_threads=16)
+    in_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e1 = B.Ensemble.from_text(text, select="ca", n_threads=1)
+    in1_s = time.perf_counter() - t0
+    small = synthetic_cif(20)
+    t0 = time.perf_counter()
+    o = I.ensemble(small, selection="ca")
+    o_s = time.perf_counter() - t0
+    ok = bool(np.array_equal(e.xyz, e1.xyz) and np.array_equal(B.Ensemble.from_text(small, select="ca").xyz, o.xyz))
+    out["ingestion"] = {
+        "workload": f"mmCIF text of {args.ingest_models} models x 255 atoms ({len(text) / 1e6:.1f} MB), "
+                    "CA selection -> (models, 15, 3) f64 (bs_ensemble_parse)",
+        "ms": round(in_s * 1e3, 2),
+        "value": round(len(text) / in_s / 1e6, 1),
+        "unit": "MB/s",
+        "threads": 16,
+        "single_thread_MBps": round(len(text) / in1_s / 1e6, 1),
+        "matches_oracle": ok,
+        "cpu_baseline": {"value": round(len(small) / o_s / 1e6, 2), "unit": "MB/s", "cores": 1, "kind": "port",
+                         "sample": f"oracle (Python restatement of the reference loader) on 20 models "
+                                   f"({len(small) / 1e6:.2f} MB) of the same text"},
+    }
+    log(f"ingestion: {len(text) / in_s / 1e6:.0f} MB/s")
+    # ---- clust TSV writer on the first 2000 elements
+    tn = 2000
+    idx = np.concatenate([i * n - i * (i + 1) // 2 + (np.arange(i + 1, tn) - i - 1) for i in range(tn)])
+    m = B.RmsdMatrix(tn, np.ascontiguousarray(sub[idx]))
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as td:
+        pth = os.path.join(td, "d.tsv")
+        C.write_tsv(m, pth, n_threads=16)
+        t0 = time.perf_counter()
+        C.write_tsv(m, pth, n_threads=16)
+        w_s = time.perf_counter() - t0
+        size = os.path.getsize(pth)
+        t0 = time.perf_counter()
+        C.write_tsv(m, pth, n_threads=1)
+        w1_s = time.perf_counter() - t0
+    out["tsv"] = {
+        "workload": f"clust TSV of the {tn}-element sub-matrix ({tn * (tn - 1) // 2} lines, {size / 1e6:.1f} MB), "
+                    "shortest round-trip f32 text (bs_write_distance_tsv)",
+        "ms": round(w_s * 1e3, 2),
+        "value": round(tn * (tn - 1) // 2 / w_s, 1),
+        "unit": "lines/s",
+        "threads": 16,
+        "cpu_baseline": {"value": round(tn * (tn - 1) // 2 / w1_s, 1), "unit": "lines/s", "cores": 1,
+                         "kind": "port", "sample": "the same writer on one thread (the oracle has no writer)"},
+    }
+    log(f"tsv: {w_s * 1e3:.1f} ms")
+    return out
+
+
 def cpu_baseline(x: np.ndarray, target_s: float, threads: int) -> dict:
     """Times the FP64 oracle on rows [0, R) with R sized for ~target_s seconds."""
     from oracle import oracle as O
@@ -244,6 +444,10 @@ def main() -> None:
     ap.add_argument("--serve-library", type=int, default=25000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-requests", type=int, default=3000)
+    ap.add_argument("--no-next", action="store_true", help="skip the clustering / superposition / "
+                    "ingestion / TSV leg")
+    ap.add_argument("--cluster-n", type=int, default=10000)
+    ap.add_argument("--ingest-models", type=int, default=2000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="copy a device buffer of this size once before the warmup (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -379,6 +583,7 @@ def main() -> None:
                       f"FP64 QCP oracle, {cb['seconds']:.1f} s",
         }
 
+    nxt = None if args.no_next else next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x)
     batch = None if args.no_batch else batch_leg(args, B, torch, log)
     serving = None if args.no_serving else serving_leg(args, B, log)
 
@@ -422,6 +627,7 @@ def main() -> None:
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serving,
+        "next": nxt,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
