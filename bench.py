@@ -444,6 +444,7 @@ def main() -> None:
     ap.add_argument("--serve-library", type=int, default=25000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-requests", type=int, default=3000)
+    ap.add_argument("--no-host-io", action="store_true", help="skip the host-buffer (PCIe-inclusive) calls")
     ap.add_argument("--no-next", action="store_true", help="skip the clustering / superposition / "
                     "ingestion / TSV leg")
     ap.add_argument("--cluster-n", type=int, default=10000)
@@ -523,13 +524,21 @@ def main() -> None:
 
     # ---------------------------------------------------------- host buffers in and out
     # (PCIe and staging included; never the headline value)
-    t0 = time.perf_counter()
-    B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=ctx)
-    host_one_shot = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    B.crmsd_all_vs_all_streamed(x, B.Method.BF16X3_GRAM, tile=4096, ctx=ctx)
-    host_streamed = time.perf_counter() - t0
-    log(f"host i/o: one-shot {host_one_shot * 1e3:.1f} ms, streamed 4096^2 {host_streamed * 1e3:.1f} ms")
+    host_io = None
+    if not args.no_host_io:
+        t0 = time.perf_counter()
+        B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=ctx)
+        host_one_shot = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        B.crmsd_all_vs_all_streamed(x, B.Method.BF16X3_GRAM, tile=4096, ctx=ctx)
+        host_streamed = time.perf_counter() - t0
+        log(f"host i/o: one-shot {host_one_shot * 1e3:.1f} ms, streamed 4096^2 {host_streamed * 1e3:.1f} ms")
+        host_io = {
+            "one_shot_ms": round(host_one_shot * 1e3, 2),
+            "streamed_4096_ms": round(host_streamed * 1e3, 2),
+            "streamed_pairs_per_s": round(pairs / host_streamed, 1),
+            "note": "host f64 coordinates in, host f32 packed triangle out (72 MB H2D, 200 MB D2H)",
+        }
 
     # ---------------------------------------------------------- roofline of the Gram kernel
     kernel_ms = float(np.mean(main_ms))
@@ -614,12 +623,7 @@ def main() -> None:
             "fallback": round(float(np.mean(fb_ms)), 4),
         },
         "flagged_pairs": int(np.mean(flagged)),
-        "host_io": {
-            "one_shot_ms": round(host_one_shot * 1e3, 2),
-            "streamed_4096_ms": round(host_streamed * 1e3, 2),
-            "streamed_pairs_per_s": round(pairs / host_streamed, 1),
-            "note": "host f64 coordinates in, host f32 packed triangle out (72 MB H2D, 200 MB D2H)",
-        },
+        "host_io": host_io,
         "plain_gpu": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "error_vs_fp64": err,
