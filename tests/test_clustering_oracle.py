@@ -134,6 +134,46 @@ def test_average_link_is_upgma_when_nothing_moves():
     assert np.allclose(np.sort(md), np.sort(Z[:, 2]), rtol=1e-5)
 
 
+@pytest.mark.parametrize("linkage", ["centroid", "median"])
+def test_centroid_and_median_rules_match_scipy_on_squared_distances(linkage):
+    """Both Lance-Williams rules are exact on squared Euclidean distances: the
+    merge heights are the squared distances between cluster centroids
+    (centroid) or between the midpoints of merged pairs (median, WPGMC), which
+    scipy computes from the points themselves.  On the halving-gap chain every
+    merge takes the last active index, so the reference's size bookkeeping
+    (sizes[] not moved) never comes into play."""
+    from scipy.cluster.hierarchy import linkage as sp_linkage
+
+    n = 18
+    x = np.cumsum(np.r_[0.0, 2.0 ** np.arange(n - 2, -1, -1)])
+    D2 = ((x[:, None] - x[None]) ** 2).astype(np.float32)
+    up = np.array([D2[i, j] for i in range(n) for j in range(i + 1, n)], dtype=np.float32)
+    mi, mj, md = O.hclust(up, n, linkage)
+    assert np.array_equal(mj, n - 1 - np.arange(n - 1))  # every merge takes the last index
+    Z = sp_linkage(x[:, None], method=linkage)
+    assert np.allclose(np.sqrt(np.sort(md.astype(np.float64))), np.sort(Z[:, 2]), rtol=2e-6)
+
+
+def test_wards_first_merge_is_textbook():
+    """The reference's Ward rule takes sizes[j] for size_k; while every
+    cluster is a singleton that equals the textbook Lance-Williams Ward
+    update, so the first merged row must be scipy's Ward distances of the
+    merged pair to each other point (squared Euclidean inputs)."""
+    rng = np.random.default_rng(4)
+    n = 12
+    P = rng.normal(size=(n, 3))
+    D2 = ((P[:, None] - P[None]) ** 2).sum(-1).astype(np.float32)
+    up = np.array([D2[i, j] for i in range(n) for j in range(i + 1, n)], dtype=np.float32)
+    mi, mj, md = O.hclust(up, n, "wards")
+    i, j = int(mi[0]), int(mj[0])
+    assert md[0] == D2[np.triu_indices(n, 1)].min()
+    # the textbook update for singletons: d(k, i+j) = (2 d_ik + 2 d_jk - d_ij) / 3
+    want = {k: (2 * D2[i, k] + 2 * D2[j, k] - D2[i, j]) / 3 for k in range(n) if k not in (i, j)}
+    # the second merge is the smallest of the updated and untouched distances
+    rest = [D2[a, b] for a in range(n) for b in range(a + 1, n) if a not in (i, j) and b not in (i, j)]
+    assert md[1] == pytest.approx(min(min(want.values()), min(rest)), rel=1e-6)
+
+
 def test_oracle_keeps_reference_quirks():
     """clust --median/--centroid map to average/wards (clust.rs:66-80); the
     merge rule is called with sizes[j] for every k and sizes are not moved
