@@ -22,6 +22,12 @@
  *     evaluated left to right without fused multiply-adds (this file is
  *     compiled with -ffp-contract=off).
  * The merge sequence (i, j, d_ij) per step is what the tree is rebuilt from.
+ *
+ * Pinned by tests/test_clustering_oracle.py: the reference's clustering tests,
+ * MST heights (single), scipy heights (single, complete; average, centroid and
+ * median on a chain where no cluster moves), the textbook first merge of
+ * Ward's rule.  Ward's later merges (which depend on the sizes[j] quirk) are
+ * parity partial: only a hand-traced case checks them.
  */
 #include <float.h>
 #include <math.h>

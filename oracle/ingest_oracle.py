@@ -37,6 +37,15 @@ Number parsing follows Rust's ``str::parse`` grammar for f64/i32/usize
 rounded ``float``/``int``).  Deposit metadata (header, entities, residue
 types, helices, sheets) is not restated: it does not change the coordinates
 or the atoms a predicate selects.
+
+Pinned by tests/test_ingest.py against the reference's own tests and
+fixtures (atom/residue/chain/model counts of 2gb1, 2fdo, 2jqb, one atom of
+1akd, PdbAtom parsing and Display, format_atom_name, is_hydrogen,
+quote_style, the ';' line iterator, the CIF reader's edge cases) and by
+2gb1.cif and 2gb1.pdb giving identical coordinates.  Parity unpinned (no
+reference test reaches them): Unicode whitespace and typographic quotes in
+CIF tokens, loop rows spanning lines or holding ';' fields, _atom_site given
+as items, the stop at a line that is not UTF-8.
 """
 from __future__ import annotations
 
