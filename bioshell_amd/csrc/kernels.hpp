@@ -100,7 +100,11 @@ constexpr int kFlagListCap = 1 << 20;  // queued FP64 re-evaluations per call (8
 void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s);
 
 // Per-query top-k over a dense n_q x n_lib score matrix (ascending, ties by index).
-void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, float* val,
+// Per-query k smallest of an n_q x n_lib score matrix (k <= min(1024, n_lib)),
+// sorted by (value, index): idx and val get n_q x k each.  scratch holds
+// topk_scratch_bytes(n_q, n_lib, k).
+size_t topk_scratch_bytes(int n_q, int n_lib, int k);
+void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, float* val, void* scratch,
                  hipStream_t s);
 
 // ------------------------------------------------------------ batches

@@ -15,10 +15,16 @@ using namespace bs;
 
 namespace bs {
 
+// Top-k results (n_q x k indices, then values) followed by the selection's candidates.
+size_t topk_bytes(int n_q, int n_lib, int k)
+{
+    return (size_t)n_q * k * 8 + (k > 0 ? topk_scratch_bytes(n_q, n_lib, k) : 0);
+}
+
 size_t query_workspace_bytes(int n_q, int n_lib, int L, int k)
 {
     return prepared_bytes(n_q, L) + (size_t)n_q * L * 3 * 8 + (size_t)n_q * n_lib * 4 +
-           (size_t)n_q * k * 8 + 4096;
+           topk_bytes(n_q, n_lib, k) + 4096;
 }
 
 void QueryWorkspace::bind(Budget* b)
@@ -49,7 +55,7 @@ void QueryWorkspace::reserve(int n_q, int n_lib, int L, int k)
     prep.consts.ensure((size_t)kNumConsts * n_pad * 8, "constants");
     xyz.ensure((size_t)n_q * L * 3 * 8, "query coordinates");
     scores.ensure((size_t)n_q * n_lib * 4, "query scores");
-    topk.ensure((size_t)n_q * k * 8, "top-k");
+    topk.ensure(topk_bytes(n_q, n_lib, k), "top-k");
     const uint64_t n_words = (((uint64_t)n_q * n_lib + 31) / 32 + 3) / 4 * 4;
     gsc.flags.ensure((size_t)(n_words + 4) * 4 + (size_t)kFlagListCap * 8, "flag bitmap and queue");
 }
@@ -79,9 +85,9 @@ void enqueue_query(bs_library* lib, QueryWorkspace& ws, const double* q_host, in
     ws.gsc.variant = lib->ctx->gsc.variant;
     run_gram(view_of(ws.prep), view_of(lib->P), run, ws.gsc, s, rec);
     if (k > 0) {
-        uint32_t* idx = static_cast<uint32_t*>(ws.topk.ensure((size_t)n_q * k * 8, "top-k"));
+        uint32_t* idx = static_cast<uint32_t*>(ws.topk.ensure(topk_bytes(n_q, n_lib, k), "top-k"));
         float* val = reinterpret_cast<float*>(idx + (size_t)n_q * k);
-        launch_topk(scores, n_q, n_lib, k, idx, val, s);
+        launch_topk(scores, n_q, n_lib, k, idx, val, idx + (size_t)n_q * k * 2, s);
         BS_HIP(hipGetLastError());
     }
     if (rec) rec->stop(kRecTotal, s);

@@ -70,6 +70,25 @@ def test_topk_ties_resolve_to_lower_index(gpu_ctx):
     lib.close()
 
 
+@pytest.mark.parametrize("n_base,copies,k", [(7000, 3, 1), (7000, 3, 100), (7000, 3, 1024), (8193, 1, 100),
+                                             (2731, 3, 1000)])
+def test_topk_selection_is_exact_over_chunks(gpu_ctx, n_base, copies, k):
+    """The selection against the device's own score rows: the k smallest by
+    (value, index), i.e. a stable argsort, across chunk boundaries and with
+    every value repeated in several chunks."""
+    base = B.generate_ensemble(17, n_base, 24, sigma=(0.5, 3.0))
+    lib_x = np.concatenate([base] * copies)
+    lib = B.Library(lib_x, ctx=gpu_ctx)
+    q = np.concatenate([base[:3], B.generate_ensemble(18, 3, 24, sigma=(0.5, 3.0))])
+    scores = lib.query_all(q)
+    idx, val = lib.query_topk(q, k)
+    for a in range(q.shape[0]):
+        want = np.argsort(scores[a], kind="stable")[:k]
+        assert np.array_equal(idx[a], want.astype(np.uint32)), a
+        assert np.array_equal(val[a].view(np.uint32), scores[a, want].view(np.uint32))
+    lib.close()
+
+
 def test_topk_argument_errors(gpu_ctx, bucket):
     lib_x, q, _ = bucket
     lib = B.Library(lib_x[:50], ctx=gpu_ctx)
