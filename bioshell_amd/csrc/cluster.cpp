@@ -31,24 +31,55 @@ void cluster_device(bs_context* ctx, const float* d_upper, size_t n, int rule, u
     if (n < 2) return;
     hipStream_t s = ctx->stream;
     const int N = (int)n;
-    const int n_part = std::max(1, std::min(2 * ctx->n_cu, (N + 3) / 4));
+    const int n_part = std::max(1, std::min(4 * ctx->n_cu, (N + 3) / 4));
     DevBuf D, aux;
     D.bind(&ctx->budget);
     aux.bind(&ctx->budget);
-    float* dm = static_cast<float*>(D.ensure(n * n * 4, "clustering matrix"));
+    const size_t ld = (n + 63) / 64 * 64;  // 256-byte rows: the scan reads 16-byte groups
+    float* dm = static_cast<float*>(D.ensure(n * ld * 4, "clustering matrix"));
     const size_t cand = cluster_candidate_bytes();
-    char* a = static_cast<char*>(aux.ensure(n * 4 * 5 + 64 + (size_t)n_part * cand, "clustering state"));
+    char* a = static_cast<char*>(aux.ensure(n * 4 * 4 + 256 + ld * 4 + 256 + (size_t)n_part * cand,
+                                            "clustering state"));
     unsigned* sizes = reinterpret_cast<unsigned*>(a);
-    float* res = reinterpret_cast<float*>(a + n * 4);
-    unsigned* mi = reinterpret_cast<unsigned*>(a + n * 8);
-    unsigned* mj = reinterpret_cast<unsigned*>(a + n * 12);
-    float* md = reinterpret_cast<float*>(a + n * 16);
-    int* state = reinterpret_cast<int*>(a + n * 20);
-    void* part = a + n * 20 + 64;
-    launch_cluster_init(d_upper, N, dm, sizes, state, s);
-    for (int step = 0; step < N - 1; ++step)
-        launch_cluster_step(dm, N, sizes, state, part, n_part, rule, res, mi, mj, md, s);
+    unsigned* mi = reinterpret_cast<unsigned*>(a + n * 4);
+    unsigned* mj = reinterpret_cast<unsigned*>(a + n * 8);
+    float* md = reinterpret_cast<float*>(a + n * 12);
+    float* rowj = reinterpret_cast<float*>(a + (n * 16 + 255) / 256 * 256);  // 16-byte aligned
+    void* state = reinterpret_cast<char*>(rowj) + ld * 4;
+    void* part = static_cast<char*>(state) + 256;
+    BS_REQUIRE(cluster_state_bytes() <= 256, "merge state too large");
+    launch_cluster_init(d_upper, N, (int)ld, dm, sizes, state, s);
     BS_HIP(hipGetLastError());
+    // the steps are identical launches: capture a run of them once and replay
+    // it (host launch cost, not the GPU, bounds a launch-per-step loop)
+    const int per_graph = std::min(128, N - 1);
+    struct ExecGuard {  // destroyed once the stream has drained
+        hipGraphExec_t exec = nullptr;
+        hipStream_t s = nullptr;
+        ~ExecGuard()
+        {
+            if (exec) {
+                (void)hipStreamSynchronize(s);
+                (void)hipGraphExecDestroy(exec);
+            }
+        }
+    } guard_exec;
+    guard_exec.s = s;
+    hipGraph_t graph = nullptr;
+    BS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int k = 0; k < per_graph; ++k)
+            launch_cluster_step(dm, N, (int)ld, state, rowj, part, n_part, sizes, rule, mi, mj, md, s);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    BS_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t ie = hipGraphInstantiate(&guard_exec.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BS_HIP(ie);
+    for (int done = 0; done < N - 1; done += per_graph) BS_HIP(hipGraphLaunch(guard_exec.exec, s));
     BS_HIP(hipMemcpyAsync(merge_i, mi, (n - 1) * 4, hipMemcpyDeviceToHost, s));
     BS_HIP(hipMemcpyAsync(merge_j, mj, (n - 1) * 4, hipMemcpyDeviceToHost, s));
     BS_HIP(hipMemcpyAsync(merge_d, md, (n - 1) * 4, hipMemcpyDeviceToHost, s));

@@ -119,66 +119,16 @@ __device__ inline bool better(const Cand& a, const Cand& b)
     return a.i < b.i;
 }
 
-__global__ void dense_from_upper_kernel(const float* __restrict__ up, int n, float* __restrict__ D,
-                                        unsigned* __restrict__ sizes, int* __restrict__ state)
+__global__ void dense_from_upper_kernel(const float* __restrict__ up, int n, int ld, float* __restrict__ D,
+                                        unsigned* __restrict__ sizes)
 {
-    const int64_t nn = (int64_t)n * n;
+    const int64_t nn = (int64_t)n * ld;
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < nn; k += (int64_t)gridDim.x * 256) {
-        const int r = (int)(k / n), c = (int)(k - (int64_t)r * n);
-        D[k] = r == c ? 0.0f : (r < c ? up[tri(n, r, c)] : up[tri(n, c, r)]);
+        const int r = (int)(k / ld), c = (int)(k - (int64_t)r * ld);
+        D[k] = c >= n ? 0.0f : (r == c ? 0.0f : (r < c ? up[tri(n, r, c)] : up[tri(n, c, r)]));
     }
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
         sizes[k] = 1u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        state[0] = n;  // order
-        state[1] = 0;  // step
-    }
-}
-
-// Closest pair, stage 1: the matrix stays bitwise symmetric on the active
-// part, so column j above the diagonal equals the row prefix D[j][0..j):
-// wave w scans rows j = 1 + w, 1 + w + W, ...; the block keeps its best.
-__global__ __launch_bounds__(256) void closest_partial_kernel(const float* __restrict__ D, int n,
-                                                              const int* __restrict__ state,
-                                                              Cand* __restrict__ part)
-{
-    __shared__ Cand wbest[4];
-    const int order = state[0];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int W = gridDim.x * 4;
-    Cand best{3.402823466e+38f, 0x7FFFFFFF, 0x7FFFFFFF};
-    for (int j = 1 + blockIdx.x * 4 + w; j < order; j += W) {
-        const float* row = D + (int64_t)j * n;
-        float v = 3.402823466e+38f;
-        int bi = 0x7FFFFFFF;
-        for (int i = lane; i < j; i += 64) {
-            const float x = row[i];
-            if (x < v) {
-                v = x;
-                bi = i;
-            }
-        }
-        // lanes: smallest value, then smallest i
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const float ov = __shfl_xor(v, o);
-            const int oi = __shfl_xor(bi, o);
-            if (ov < v || (ov == v && oi < bi)) {
-                v = ov;
-                bi = oi;
-            }
-        }
-        const Cand c{v, j, bi};
-        if (bi != 0x7FFFFFFF && better(c, best)) best = c;
-    }
-    if (lane == 0) wbest[w] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Cand b = wbest[0];
-        for (int k = 1; k < 4; ++k)
-            if (better(wbest[k], b)) b = wbest[k];
-        part[blockIdx.x] = b;
-    }
 }
 
 // The merging rules of strategies/mod.rs in f32, evaluated left to right with
@@ -207,81 +157,203 @@ __device__ inline float merge_rule(int rule, unsigned si, unsigned sj, unsigned 
     }
 }
 
-// Closest pair, stage 2, and the merge -- one block:
-//   the best of the partial candidates is the pair (i, j), i < j;
-//   update_distances(i, j, rule, i): rule(size_i, size_j, sizes[j], d_ij,
-//     D[i][k], D[j][k]) for every active k, computed before any is written,
-//     then written to row and column i; D[i][i] = 0; sizes[i] = size_j + size_i
-//     (clustering_matrix.rs:57-74, hierarchical.rs:66);
-//   replace_with_last(j) if j is not the last active index: row j takes the
-//     last row, then column j the last column (clustering_matrix.rs:47-55);
-//     sizes[] is not moved, as in the reference; the order drops by one.
-__global__ __launch_bounds__(1024) void merge_step_kernel(float* __restrict__ D, int n,
-                                                          unsigned* __restrict__ sizes,
-                                                          int* __restrict__ state,
-                                                          const Cand* __restrict__ part, int n_part,
-                                                          int rule, float* __restrict__ res,
-                                                          unsigned* __restrict__ mi,
-                                                          unsigned* __restrict__ mj,
-                                                          float* __restrict__ md)
+// The decision of one merge, written by the last work-group of a launch and
+// applied by every work-group of the next.
+struct MergeState {
+    int s;             // the next merge to find (0 .. n - 2); n - 1 when done
+    int i, j;          // the pair, i < j (or the reference's (0, 0))
+    int order;         // active elements before the merge
+    unsigned si, sj;   // sizes[i], sizes[j] before the merge
+    float dij;
+};
+
+// Merges are applied and found by two launches per merge:
+// cluster_step_kernel (every work-group) applies the previous merge, decided
+// by the previous cluster_decide_kernel, and finds each work-group's best
+// candidate; cluster_decide_kernel (one work-group) takes the best of them.
+//   update_distances(i, j, rule, i): res[k] = rule(size_i, size_j, sizes[j],
+//     d_ij, D[i][k], D[j][k]), written to row and column i, D[i][i] = 0
+//     (clustering_matrix.rs:57-74);
+//   replace_with_last(j) when j is not the last active index: row and column
+//     j take the last ones (clustering_matrix.rs:47-55); sizes[] not moved;
+//   closest pair: the first strict minimum of the scan j ascending, i
+//     ascending (clustering_matrix.rs:27-42), i.e. the smallest (value, j, i)
+//     over values below f32::MAX, else (0, 0).
+// Each wave owns rows r = W k + wave id: it applies the merge to its own rows
+// and then scans their prefixes D[r][0..r) (= column r above the diagonal),
+// four floats per load.  By symmetry the update of row r needs only its own
+// old D[r][i], D[r][j], D[r][last]; the owner of row i also needs the old row
+// j, which the owner of row j overwrites meanwhile -- the decide kernel keeps a
+// copy of it.  Nobody writes the last row or column (it leaves the active
+// part), so they are read in place.  No wave reads another's rows otherwise,
+// so the update and the scan need no barrier between work-groups.
+__global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D, int n, int ld,
+                                                           const MergeState* __restrict__ st,
+                                                           const float* __restrict__ rowj,
+                                                           Cand* __restrict__ part, int rule)
+{
+    __shared__ Cand wbest[4];
+    const int s = st->s;
+    if (s >= n - 1) return;  // every merge found: a spare step of the graph
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int W = gridDim.x * 4, wid = blockIdx.x * 4 + w;
+    const int order = n - s;  // active elements after the previous merge
+    const float* rlast = D + (int64_t)order * ld;  // the previous last row (index `order`)
+    int mi_ = -1, mj_ = -1;
+    bool moved = false;
+    unsigned msi = 0, msj = 0;
+    float mdij = 0.0f;
+    if (s > 0) {
+        mi_ = st->i;
+        mj_ = st->j;
+        moved = mj_ < order;  // j below the previous order's last index (= order)
+        msi = st->si;
+        msj = st->sj;
+        mdij = st->dij;
+    }
+    Cand best{3.402823466e+38f, 0x7FFFFFFF, 0x7FFFFFFF};
+    for (int r = wid; r < order; r += W) {
+        float* row = D + (int64_t)r * ld;
+        if (s > 0) {
+            // res[last] (the merged cluster to the element that moves to j)
+            // comes from column `order` of rows i and j, which nobody writes
+            if (moved && r == mj_) {  // first: with the (0, 0) pair row i is row j
+                const float rl = merge_rule(rule, msi, msj, msj, mdij, D[(int64_t)mi_ * ld + order],
+                                            D[(int64_t)mj_ * ld + order]);
+                for (int c = lane; c < order; c += 64) row[c] = c == mj_ ? 0.0f : (c == mi_ ? rl : rlast[c]);
+            } else if (r == mi_) {
+                const float rl = moved ? merge_rule(rule, msi, msj, msj, mdij, row[order], rowj[order]) : 0.0f;
+                for (int c = lane; c < order; c += 64) {
+                    float v;
+                    if (c == mi_)
+                        v = 0.0f;
+                    else if (moved && c == mj_)
+                        v = rl;
+                    else
+                        v = merge_rule(rule, msi, msj, msj, mdij, row[c], rowj[c]);
+                    row[c] = v;
+                }
+            } else if (lane == 0) {
+                const float vi = row[mi_], vj = row[mj_], vl = row[order];
+                row[mi_] = merge_rule(rule, msi, msj, msj, mdij, vi, vj);
+                if (moved) row[mj_] = vl;
+            }
+            // the lanes' writes to this row are read back by the scan below
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (r == 0) continue;
+        // smallest value of the prefix, first index on ties: lane l takes the
+        // 4-float groups l, l + 64, ... (rows are 16-byte aligned, ld % 4 == 0)
+        float v = 3.402823466e+38f;
+        int bi = 0x7FFFFFFF;
+        const int r4 = r >> 2;
+        const float4* row4 = reinterpret_cast<const float4*>(row);
+        int g = lane;
+        for (; g + 192 < r4; g += 256) {
+            const float4 x0 = row4[g], x1 = row4[g + 64], x2 = row4[g + 128], x3 = row4[g + 192];
+            const float4 xs[4] = {x0, x1, x2, x3};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c0 = 4 * (g + 64 * u);
+                if (xs[u].x < v) { v = xs[u].x; bi = c0; }
+                if (xs[u].y < v) { v = xs[u].y; bi = c0 + 1; }
+                if (xs[u].z < v) { v = xs[u].z; bi = c0 + 2; }
+                if (xs[u].w < v) { v = xs[u].w; bi = c0 + 3; }
+            }
+        }
+        for (; g < r4; g += 64) {
+            const float4 x = row4[g];
+            const int c0 = 4 * g;
+            if (x.x < v) { v = x.x; bi = c0; }
+            if (x.y < v) { v = x.y; bi = c0 + 1; }
+            if (x.z < v) { v = x.z; bi = c0 + 2; }
+            if (x.w < v) { v = x.w; bi = c0 + 3; }
+        }
+        const int c_tail = 4 * r4 + lane;
+        if (c_tail < r) {
+            const float x = row[c_tail];
+            if (x < v) { v = x; bi = c_tail; }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(v, o);
+            const int ob = __shfl_xor(bi, o);
+            if (ov < v || (ov == v && ob < bi)) {
+                v = ov;
+                bi = ob;
+            }
+        }
+        const Cand c{v, r, bi};
+        if (bi != 0x7FFFFFFF && better(c, best)) best = c;
+    }
+    if (lane == 0) wbest[w] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cand b = wbest[0];
+        for (int k = 1; k < 4; ++k)
+            if (better(wbest[k], b)) b = wbest[k];
+        part[blockIdx.x] = b;
+    }
+}
+
+// The closest pair over the work-groups' candidates: records the merge, sets
+// sizes[i] = size_j + size_i (hierarchical.rs:66) and copies the old row j for
+// the next update.
+__global__ __launch_bounds__(1024) void cluster_decide_kernel(const float* __restrict__ D, int n, int ld,
+                                                              MergeState* __restrict__ st,
+                                                              float* __restrict__ rowj,
+                                                              const Cand* __restrict__ part, int n_part,
+                                                              unsigned* __restrict__ sizes,
+                                                              unsigned* __restrict__ mi,
+                                                              unsigned* __restrict__ mj,
+                                                              float* __restrict__ md)
 {
     __shared__ Cand red[1024];
     __shared__ int s_i, s_j;
-    __shared__ float s_d;
-    const int t = threadIdx.x;
-    const int order = state[0];
-    if (order < 2) return;  // every block sees the same order
+    const int s = st->s;  // read by every thread before thread 0 advances it below
+    if (s >= n - 1) return;
+    const int order = n - s;
     Cand b{3.402823466e+38f, 0x7FFFFFFF, 0x7FFFFFFF};
-    for (int k = t; k < n_part; k += 1024)
-        if (better(part[k], b)) b = part[k];
-    red[t] = b;
+    for (int k = threadIdx.x; k < n_part; k += 1024) {
+        const Cand c = part[k];
+        if (better(c, b)) b = c;
+    }
+    red[threadIdx.x] = b;
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s && better(red[t + s], red[t])) red[t] = red[t + s];
+    for (int h = 512; h > 0; h >>= 1) {
+        if (threadIdx.x < h && better(red[threadIdx.x + h], red[threadIdx.x])) red[threadIdx.x] = red[threadIdx.x + h];
         __syncthreads();
     }
-    if (t == 0) {
+    if (threadIdx.x == 0) {
         Cand c = red[0];
-        if (c.j == 0x7FFFFFFF) {  // every value >= f32::MAX: the reference returns (0, 0)
+        if (c.j == 0x7FFFFFFF) {  // every value >= f32::MAX: the reference keeps (0, 0)
             c.i = 0;
             c.j = 0;
         }
         s_i = c.i;
         s_j = c.j;
-        s_d = D[(int64_t)c.i * n + c.j];
-        const int step = state[1];
-        mi[step] = (unsigned)c.i;
-        mj[step] = (unsigned)c.j;
-        md[step] = s_d;
     }
     __syncthreads();
     const int i = s_i, j = s_j;
-    const float dij = s_d;
-    const unsigned si = sizes[i], sj = sizes[j];
-    const float* ri = D + (int64_t)i * n;
-    const float* rj = D + (int64_t)j * n;
-    for (int k = t; k < order; k += 1024) res[k] = merge_rule(rule, si, sj, sj, dij, ri[k], rj[k]);
-    __syncthreads();
-    for (int k = t; k < order; k += 1024) {
-        D[(int64_t)i * n + k] = res[k];
-        D[(int64_t)k * n + i] = res[k];
-    }
-    __syncthreads();
-    if (t == 0) {
-        D[(int64_t)i * n + i] = 0.0f;
+    // old row j, columns 0 .. order - 1 (the next update reads up to index order - 1)
+    const float4* src = reinterpret_cast<const float4*>(D + (int64_t)j * ld);
+    float4* dst = reinterpret_cast<float4*>(rowj);
+    for (int k = threadIdx.x; k < (order + 3) / 4; k += 1024) dst[k] = src[k];
+    if (threadIdx.x == 0) {
+        const unsigned si = sizes[i], sj = sizes[j];
+        const float dij = D[(int64_t)i * ld + j];
+        st->i = i;
+        st->j = j;
+        st->si = si;
+        st->sj = sj;
+        st->dij = dij;
         sizes[i] = sj + si;
-    }
-    __syncthreads();
-    const int last = order - 1;
-    if (j < last) {
-        for (int c = t; c <= last; c += 1024) D[(int64_t)j * n + c] = D[(int64_t)last * n + c];
-        __syncthreads();
-        for (int r = t; r < last; r += 1024) D[(int64_t)r * n + j] = D[(int64_t)r * n + last];
-        __syncthreads();
-    }
-    if (t == 0) {
-        state[0] = last;
-        state[1] += 1;
+        mi[s] = (unsigned)i;
+        mj[s] = (unsigned)j;
+        md[s] = dij;
+        st->s = s + 1;
     }
 }
 
@@ -304,21 +376,24 @@ void launch_members_row_max(const float* up, int n, const unsigned* members, int
     members_row_max_kernel<<<m, 256, 0, s>>>(up, n, members, m, out);
 }
 
-void launch_cluster_init(const float* up, int n, float* D, unsigned* sizes, int* state,
-                         hipStream_t s)
+void launch_cluster_init(const float* up, int n, int ld, float* D, unsigned* sizes, void* state, hipStream_t s)
 {
-    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(4096, ((int64_t)n * n + 255) / 256));
-    dense_from_upper_kernel<<<(int)nb, 256, 0, s>>>(up, n, D, sizes, state);
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(4096, ((int64_t)n * ld + 255) / 256));
+    dense_from_upper_kernel<<<(int)nb, 256, 0, s>>>(up, n, ld, D, sizes);
+    fill_u32_kernel<<<1, 256, 0, s>>>(static_cast<unsigned*>(state), 0u,
+                                      (int64_t)(sizeof(MergeState) / sizeof(unsigned)));
 }
 
-void launch_cluster_step(float* D, int n, unsigned* sizes, int* state, void* part, int n_part,
-                         int rule, float* res, unsigned* mi, unsigned* mj, float* md, hipStream_t s)
+void launch_cluster_step(float* D, int n, int ld, void* state, float* rowj, void* part, int n_part,
+                         unsigned* sizes, int rule, unsigned* mi, unsigned* mj, float* md, hipStream_t s)
 {
-    closest_partial_kernel<<<n_part, 256, 0, s>>>(D, n, state, static_cast<Cand*>(part));
-    merge_step_kernel<<<1, 1024, 0, s>>>(D, n, sizes, state, static_cast<const Cand*>(part), n_part,
-                                         rule, res, mi, mj, md);
+    auto* st = static_cast<MergeState*>(state);
+    auto* cand = static_cast<Cand*>(part);
+    cluster_step_kernel<<<n_part, 256, 0, s>>>(D, n, ld, st, rowj, cand, rule);
+    cluster_decide_kernel<<<1, 1024, 0, s>>>(D, n, ld, st, rowj, cand, n_part, sizes, mi, mj, md);
 }
 
+size_t cluster_state_bytes() { return sizeof(MergeState); }
 size_t cluster_candidate_bytes() { return sizeof(Cand); }
 
 }  // namespace bs
