@@ -33,24 +33,51 @@ sys.path.insert(0, ROOT)
 METRIC = "superpositions/s all-vs-all N=10k L=300 vs plain GPU; p50/p99 ms at fixed QPS"
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 PEAK_FP32_VALU_TFLOPS = 157.3
+PEAK_FP64_VALU_TFLOPS = 78.6
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_bytes(n: int, L: int) -> int | None:
-    """HBM bytes per Gram launch from the committed rocprofv3 --pmc summary of this
-    workload (FETCH_SIZE and WRITE_SIZE, each scaled by the factor the same counter
-    reads for a device copy of known size), or None when there is none for (n, L)."""
+def pmc_figure(n: int, L: int, key: str) -> float | None:
+    """A figure of the committed rocprofv3 --pmc summary of the 10,000 x 300 Gram
+    launch (a `key = value` line), or None for other shapes or when absent."""
     path = os.path.join(ROOT, "profiles", "gram_bf16x3_10k_300_pmc.txt")
     if (n, L) != (10000, 300) or not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch"):
-                return int(float(line.split("=")[1]))
+            if line.startswith(key + " "):
+                return float(line.split("=")[1].split()[0])
     return None
+
+
+def traffic_bytes(n: int, L: int) -> int | None:
+    """HBM bytes per Gram launch: FETCH_SIZE and WRITE_SIZE, each scaled by the factor
+    the same counter reads for a device copy of known size."""
+    v = pmc_figure(n, L, "traffic_bytes_per_launch")
+    return None if v is None else int(v)
+
+
+def measured_peaks() -> dict:
+    """Dense BF16 MFMA and FP64 FMA rates this GPU sustains in a probe kernel
+    (reference points only; every frac uses the published peak)."""
+    import ctypes
+
+    from bioshell_amd import _native
+
+    f = _native.lib().bsi_peak_probe
+    f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    f.restype = ctypes.c_int
+    out = {}
+    for which, key in ((0, "bf16_mfma_tflops"), (1, "fp64_fma_tflops")):
+        v = ctypes.c_double(0.0)
+        rc = f(which, ctypes.byref(v))
+        out[key] = round(v.value, 1) if rc == 0 else None
+    out["probe"] = ("back-to-back v_mfma_f32_32x32x16_bf16 on 8 accumulators per wave, and 16 FP64 FMA "
+                    "chains per lane, all CUs, best of 3 runs of ~1 ms")
+    return out
 
 
 def batch_leg(args, B, torch, log) -> dict:
@@ -555,6 +582,21 @@ def main() -> None:
         "kernel_ms": round(kernel_ms, 4),
         "issued_mfma_tflops": round(3 * achieved, 2),
     }
+    # the FP64 QCP epilogue inside the same kernel, as its own fraction
+    fp64_per_pair = pmc_figure(n, L, "fp64_flops_per_pair")
+    if fp64_per_pair is not None:
+        f64 = fp64_per_pair * pairs / (kernel_ms * 1e-3) / 1e12
+        roofline["fp64_epilogue"] = {
+            "bound": "fp64_valu", "achieved": round(f64, 2), "peak": PEAK_FP64_VALU_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(f64 / PEAK_FP64_VALU_TFLOPS, 4),
+            "flops_per_pair": fp64_per_pair,
+            "source": "SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 of the Gram launch (FMA = 2 flops), "
+                      "profiles/gram_bf16x3_10k_300_pmc.txt",
+        }
+    peaks = measured_peaks()
+    if peaks.get("bf16_mfma_tflops"):
+        peaks["gram_frac_of_measured_bf16"] = round(achieved / peaks["bf16_mfma_tflops"], 4)
+    roofline["measured_peaks"] = peaks
 
     # ---------------------------------------------------------- error and CPU baseline
     err = None
