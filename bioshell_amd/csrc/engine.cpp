@@ -205,6 +205,20 @@ void GramScratch::reset()
     order_done = nullptr;
 }
 
+void GramScratch::reserve_orders(size_t max_tiles)
+{
+    for (auto& o : orders) o.buf.ensure(max_tiles * 4 + 4, "tile order");
+    if (max_tiles > h_order_cap) {
+        if (!order_done) BS_HIP(hipEventCreateWithFlags(&order_done, hipEventDisableTiming));
+        BS_HIP(hipEventSynchronize(order_done));
+        if (h_order) BS_HIP(hipHostFree(h_order));
+        h_order = nullptr;
+        h_order_cap = 0;
+        BS_HIP(hipHostMalloc((void**)&h_order, max_tiles * 4, hipHostMallocDefault));
+        h_order_cap = max_tiles;
+    }
+}
+
 const uint32_t* GramScratch::tile_order(int tiles_r, int tiles_c, bool triangle, hipStream_t s,
                                         int* n)
 {

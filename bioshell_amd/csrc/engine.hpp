@@ -102,6 +102,10 @@ struct GramScratch {
     void reset();
     // Device tile order for the shape; uploaded on first use (stream-ordered).
     const uint32_t* tile_order(int tiles_r, int tiles_c, bool triangle, hipStream_t s, int* n);
+    // Sizes every cached order and the pinned staging for up to max_tiles
+    // tiles, so that new shapes up to that size never allocate.
+    void reserve_orders(size_t max_tiles);
+    static size_t order_bytes(size_t max_tiles) { return kOrders * (max_tiles * 4 + 4); }
 };
 
 // Events and host-visible results of one call, read lazily.

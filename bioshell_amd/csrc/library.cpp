@@ -21,10 +21,16 @@ size_t topk_bytes(int n_q, int n_lib, int k)
     return (size_t)n_q * k * 8 + (k > 0 ? topk_scratch_bytes(n_q, n_lib, k) : 0);
 }
 
+// Tiles of a query batch: 64-row tiles of the queries x 64-column tiles of the library.
+static size_t query_tiles(int n_q, int n_lib)
+{
+    return (size_t)((n_q + 63) / 64) * (size_t)((n_lib + 63) / 64);
+}
+
 size_t query_workspace_bytes(int n_q, int n_lib, int L, int k)
 {
     return prepared_bytes(n_q, L) + (size_t)n_q * L * 3 * 8 + (size_t)n_q * n_lib * 4 +
-           topk_bytes(n_q, n_lib, k) + 4096;
+           topk_bytes(n_q, n_lib, k) + GramScratch::order_bytes(query_tiles(n_q, n_lib)) + 4096;
 }
 
 void QueryWorkspace::bind(Budget* b)
@@ -58,6 +64,7 @@ void QueryWorkspace::reserve(int n_q, int n_lib, int L, int k)
     topk.ensure(topk_bytes(n_q, n_lib, k), "top-k");
     const uint64_t n_words = (((uint64_t)n_q * n_lib + 31) / 32 + 3) / 4 * 4;
     gsc.flags.ensure((size_t)(n_words + 4) * 4 + (size_t)kFlagListCap * 8, "flag bitmap and queue");
+    gsc.reserve_orders(query_tiles(n_q, n_lib));
 }
 
 // Enqueues the whole query path on stream s: H2D of the queries (from

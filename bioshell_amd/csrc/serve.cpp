@@ -65,6 +65,7 @@ struct bs_server {
     size_t budget = 0;
     size_t reserved = 0;
     int max_batch_queries = 512;
+    int merge_queries = 512;  // queries merged into one batch (the size slots are reserved for)
     std::deque<Slot> slots;  // deque: slots are never moved
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -114,7 +115,7 @@ bool bs_server::admit_locked(Slot& s, std::vector<ReqPtr>& batch)
     int n_q = 0, k = 0;
     for (const auto& r : queue) {
         if (r->lib != lib) continue;
-        if (!batch.empty() && n_q + r->n_q > max_batch_queries) break;
+        if (!batch.empty() && n_q + r->n_q > merge_queries) break;
         batch.push_back(r);
         n_q += r->n_q;
         k = std::max(k, r->k);
@@ -270,15 +271,17 @@ bs_status bs_server_create(bs_context* ctx, bs_library* const* libs, size_t n_li
         srv->libs.assign(libs, libs + n_libs);
         srv->budget = hbm_budget_bytes;
         for (int i = 0; i < max_inflight; ++i) srv->slots.emplace_back();
-        // Every slot is sized up front for a typical batch (up to 64 queries of
-        // the longest library against the largest one, k up to 1024), so that
-        // serving does not allocate, and free, device memory between requests.
+        // Every slot is sized up front for a merged batch (up to 256 queries of
+        // the longest library against the largest one, k up to 1024), and
+        // requests are merged only up to that size, so that serving does not
+        // allocate, and free, device memory between requests (a free
+        // synchronises the whole device and stalls every batch in flight).
         int max_n = 0, max_L = 0;
         for (size_t i = 0; i < n_libs; ++i) {
             max_n = std::max(max_n, libs[i]->P.n);
             max_L = std::max(max_L, libs[i]->P.L);
         }
-        const int pre_q = std::min(64, srv->max_batch_queries);
+        const int pre_q = std::min(256, srv->max_batch_queries);
         const int pre_k = std::min(1024, max_n);
         for (auto& s : srv->slots) {
             BS_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
@@ -290,6 +293,7 @@ bs_status bs_server_create(bs_context* ctx, bs_library* const* libs, size_t n_li
                 s.ws.reserve(pre_q, max_n, max_L, pre_k);
                 ensure_pinned(s.h_q, s.h_q_cap, (size_t)pre_q * max_L * 3);
                 ensure_pinned(s.h_out, s.h_out_cap, (size_t)pre_q * pre_k * 2);
+                srv->merge_queries = pre_q;
             }
         }
         bs_server* raw = srv.get();
