@@ -9,11 +9,12 @@ flagged pairs, writing the 49,995,000-value packed f32 triangle to HBM.
 
 Also measured in the same run: the plain per-pair GPU kernel (FP32) on the
 same input, the error of both against the FP64 oracle on a row sample, and the
-oracle itself on the host cores (cpu_baseline, a bounded row sample).  Two
-more legs report the other north-star workloads beside the headline line:
-"batch" (512 ensembles of uneven n and L as one graph launch, autotuned tiles)
-and "serving" (query-vs-library top-100 over device-resident libraries,
-Poisson arrivals at a fixed rate, p50/p99 latency and pairs/s).
+oracle itself on the host cores (cpu_baseline, a bounded row sample).  More
+legs report the neighbouring workloads beside the headline line: "batch" (512
+ensembles of uneven n and L as one graph launch, autotuned tiles), "serving"
+(query-vs-library top-100 over device-resident libraries, Poisson arrivals at
+a fixed rate, p50/p99 latency and pairs/s) and "next" (clustering, row
+reductions, superposition output, ingestion and the TSV writer).
 
 Prints one JSON line.
 """
