@@ -144,6 +144,28 @@ def test_server_results_equal_alone_and_in_mix(gpu_ctx):
         l.close()
 
 
+def test_server_large_requests_beside_small_ones(gpu_ctx):
+    """Requests larger than the batch size slots reserve (256 queries) run as
+    batches of their own and grow the slot; merges of small requests stop at
+    the reserved size.  Every result equals the synchronous call."""
+    lx, qq = make_bucket(140, 300, 420, 32)
+    lib = B.Library(lx, ctx=gpu_ctx)
+    sizes = [3, 400, 7, 200, 200, 1, 60, 300, 5]
+    reqs, s0 = [], 0
+    for n in sizes:
+        reqs.append(qq[s0:s0 + n] if s0 + n <= len(qq) else qq[:n])
+        s0 = (s0 + 37) % 20
+    alone = [lib.query_topk(q, 15) for q in reqs]
+    srv = B.Server([lib], max_inflight=2)
+    tickets = [srv.submit(0, q, 15) for q in reqs]
+    for r, t in enumerate(tickets):
+        idx, val = t.wait()
+        assert np.array_equal(idx, alone[r][0]), f"request {r} ({len(reqs[r])} queries)"
+        assert np.array_equal(val, alone[r][1])
+    srv.close()
+    lib.close()
+
+
 def test_server_budget_and_cancel(gpu_ctx):
     lx, qq = make_bucket(7, 600, 16, 48)
     lib = B.Library(lx, ctx=gpu_ctx)
