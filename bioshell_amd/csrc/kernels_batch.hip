@@ -80,7 +80,9 @@ __global__ __launch_bounds__(256) void batch_fit_pack_kernel(BatchArgs b)
 {
     __shared__ uint4 img[4][kPackWordsU4];
     const int wv = threadIdx.x >> 6;
-    const int g = blockIdx.x * 4 + wv;
+    const int blk = xcd_contiguous_block((b.total_pad + 3) / 4);
+    if (blk < 0) return;
+    const int g = blk * 4 + wv;
     if (g >= b.total_pad) return;
     const int e = find_ens(b.pad_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
@@ -257,7 +259,7 @@ void launch_batch_prep(const BatchArgs& b, hipStream_t s)
     batch_mean_partial_kernel<<<b.total_slices, 256, 0, s>>>(b);
     batch_mean_final_kernel<<<(b.total_coords + 255) / 256, 256, 0, s>>>(b);
     batch_ref_gram_kernel<<<b.n_ens, 256, 0, s>>>(b);
-    const int pw = (b.total_pad + 3) / 4;
+    const int pw = xcd_contiguous_grid((b.total_pad + 3) / 4);
     if (b.mode == kGramBF16x3)
         batch_fit_pack_kernel<kGramBF16x3><<<pw, 256, 0, s>>>(b);
     else

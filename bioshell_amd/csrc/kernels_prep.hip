@@ -55,7 +55,9 @@ __global__ __launch_bounds__(256) void fit_pack_kernel(const double* __restrict_
 {
     __shared__ uint4 img[4][kPackWordsU4];
     const int wv = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + wv;  // grid covers n_pad exactly
+    const int blk = xcd_contiguous_block((n_pad + 3) / 4);
+    if (blk < 0) return;
+    const int i = blk * 4 + wv;  // the logical blocks cover n_pad exactly
     if (i >= n_pad) return;
     fit_pack_one<MODE>(xyz, ref, n, L, n_kblk, i, rot, cent, panels, consts, ld, img[wv],
                        threadIdx.x & 63);
@@ -120,7 +122,7 @@ void launch_fit_pack(const double* xyz, const double* ref, int n, int n_pad, int
                      GramMode mode, double* rot, double* cent, void* panels, double* consts, int ld,
                      hipStream_t s)
 {
-    const int blocks = (n_pad + 3) / 4;
+    const int blocks = xcd_contiguous_grid((n_pad + 3) / 4);
     uint4* p = static_cast<uint4*>(panels);
     if (mode == kGramBF16x3)
         fit_pack_kernel<kGramBF16x3><<<blocks, 256, 0, s>>>(xyz, ref, n, n_pad, L, n_kblk, rot,

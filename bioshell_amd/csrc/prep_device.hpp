@@ -269,6 +269,20 @@ __device__ inline void mean_final_wave(const double* __restrict__ partial, int n
     if (lane == 0) ref[v] = acc / n;
 }
 
+// Work-group ids are dealt to the 8 XCDs in turn.  A pack grid of 8 * per
+// work-groups maps id b to the logical block (b % 8) * per + b / 8, so that
+// each XCD packs a contiguous range of conformations: the 16-byte panel words
+// of one conformation block (32 conformations, 8 work-groups of 4 waves) are
+// then written through one L2, which merges them into whole lines, instead of
+// arriving as partial lines from eight.  Returns -1 for the padding ids.
+__device__ inline int xcd_contiguous_block(int n_blocks)
+{
+    const int per = (n_blocks + 7) / 8;
+    const int lb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    return lb < n_blocks ? lb : -1;
+}
+inline int xcd_contiguous_grid(int n_blocks) { return (n_blocks + 7) / 8 * 8; }
+
 // Fit + pack: one wave per conformation i of one ensemble.  Pass 1 reduces
 // sum x_k and M = sum x_k r_k^T (raw coordinates; r is centred) and turns M
 // into U_i (every lane).  Pass 2 (a second read of x, mostly from cache)
