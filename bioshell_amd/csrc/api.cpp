@@ -69,6 +69,7 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         ctx->ws_xyz.bind(&ctx->budget);
         ctx->ws_out.bind(&ctx->budget);
         ctx->ws_misc.bind(&ctx->budget);
+        ctx->ws_stats.bind(&ctx->budget);
         ctx->batch.bind(&ctx->budget);
         *out = ctx.release();
     });
@@ -98,6 +99,7 @@ void bs_context_destroy(bs_context* ctx)
         ctx->ws_xyz.reset();
         ctx->ws_out.reset();
         ctx->ws_misc.reset();
+        ctx->ws_stats.reset();
         ctx->batch.release();
         (void)hipStreamDestroy(ctx->stream);
     });

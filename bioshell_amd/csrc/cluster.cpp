@@ -98,7 +98,8 @@ bs_status bs_rmsd_row_stats_device(bs_context* ctx, const float* d_upper, size_t
         BS_REQUIRE((n < 2 || d_upper) && d_row_min && d_row_max, "null pointer argument");
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
-        launch_row_stats(d_upper, (int)n, d_row_min, d_row_max, ctx->stream);
+        void* scratch = ctx->ws_stats.ensure(row_stats_scratch_bytes((int)n), "row statistics tiles");
+        launch_row_stats(d_upper, (int)n, d_row_min, d_row_max, scratch, ctx->stream);
         BS_HIP(hipGetLastError());
     });
 }
@@ -118,7 +119,8 @@ bs_status bs_rmsd_row_stats(bs_context* ctx, const float* upper, size_t n, float
         }
         const float* d = upload_upper(ctx, upper, n);
         float* st = static_cast<float*>(ctx->ws_misc.ensure(n * 8, "row statistics"));
-        launch_row_stats(d, (int)n, st, st + n, ctx->stream);
+        void* scratch = ctx->ws_stats.ensure(row_stats_scratch_bytes((int)n), "row statistics tiles");
+        launch_row_stats(d, (int)n, st, st + n, scratch, ctx->stream);
         BS_HIP(hipGetLastError());
         BS_HIP(hipMemcpyAsync(row_min, st, n * 4, hipMemcpyDeviceToHost, ctx->stream));
         BS_HIP(hipMemcpyAsync(row_max, st + n, n * 4, hipMemcpyDeviceToHost, ctx->stream));

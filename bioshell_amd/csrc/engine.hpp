@@ -162,6 +162,7 @@ struct bs_context {
     bs::Prepared prep;         // the last ensemble's prepared set
     bs::GramScratch gsc;
     bs::DevBuf ws_xyz, ws_out, ws_misc;
+    bs::DevBuf ws_stats;  // per-tile partial results of the row statistics
     hipStream_t copy_stream = nullptr;  // device-to-host copies overlapping the compute stream
     bs::DevBuf stream_blk[2];          // streamed all-vs-all: result blocks on the device
     void* h_blk[2] = {nullptr, nullptr};  // pinned result blocks

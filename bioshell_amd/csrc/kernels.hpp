@@ -163,8 +163,11 @@ void launch_batch_gram(const BatchArgs& b, uint32_t* zero_base, int64_t zero_wor
 void launch_batch_fallback(const BatchArgs& b, hipStream_t s);
 
 // ------------------------------------------------------------ clustering
-// Row minimum and maximum over j != i of a packed triangle (distances >= 0).
-void launch_row_stats(const float* up, int n, float* rmin, float* rmax, hipStream_t s);
+// Row minimum and maximum over j != i of a packed triangle, as scans from
+// f32::MAX and -f32::MAX with `d < min` / `d > max` give them (NaN never
+// counts); scratch holds row_stats_scratch_bytes(n).
+size_t row_stats_scratch_bytes(int n);
+void launch_row_stats(const float* up, int n, float* rmin, float* rmax, void* scratch, hipStream_t s);
 // For each of m members, the largest distance to the other members.
 void launch_members_row_max(const float* up, int n, const unsigned* members, int m, float* out,
                             hipStream_t s);

@@ -13,6 +13,7 @@
 // comments at each phase); the merge sequence it records is what the host
 // rebuilds the tree from.
 #include "kernels.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 
@@ -33,50 +34,104 @@ __global__ void fill_u32_kernel(unsigned* p, unsigned v, int64_t n)
         p[k] = v;
 }
 
-// Block (tr, tc), tr <= tc: rows 64 tr.., columns 64 tc..; thread t takes row
-// t / 4 and 16 columns from 16 (t % 4).
-__global__ __launch_bounds__(256) void row_stats_kernel(const float* __restrict__ up, int n,
-                                                        unsigned* __restrict__ rmin,
-                                                        unsigned* __restrict__ rmax)
+// Row statistics compare f32 values through order-preserving unsigned keys
+// (every non-NaN float, negative ones included, keeps its order); NaN entries
+// are skipped, as the reference's `d < min` / `d > max` tests skip them.
+__device__ inline unsigned fkey(unsigned bits) { return (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u); }
+__device__ inline unsigned fkey_inverse(unsigned key) { return (key & 0x80000000u) ? (key ^ 0x80000000u) : ~key; }
+constexpr unsigned kKeyFltMax = 0xFF7FFFFFu;     // fkey(bits of f32::MAX)
+constexpr unsigned kKeyNegFltMax = 0x00800000u;  // fkey(bits of -f32::MAX)
+
+// First block of tile row r in the row-by-row order of the T x T upper
+// triangle of tiles (tr <= tc).
+__host__ __device__ inline int tile_row_first(int r, int T) { return r * T - r * (r - 1) / 2; }
+
+// Block b is tile (tr, tc), tr <= tc: rows 64 tr.., columns 64 tc.. of the
+// packed triangle.  Wave w takes rows 64 tr + 16 w .. + 15; lane l holds
+// column 64 tc + l, so every row's 64 values are one 256-byte read, and the
+// wave issues its 16 reads before using any.  The tile's results go to
+// part[256 b ..]: row minima, row maxima, column minima, column maxima (64
+// keys each; a row or column with no entry here holds the identity).
+__global__ __launch_bounds__(256) void row_stats_tile_kernel(const float* __restrict__ up, int n, int T,
+                                                             unsigned* __restrict__ part)
 {
-    const int tr = blockIdx.y, tc = blockIdx.x;
-    if (tr > tc) return;
-    __shared__ unsigned cmin[64], cmax[64], wmin[64], wmax[64];
-    const int t = threadIdx.x;
-    if (t < 64) {
-        cmin[t] = 0x7F800000u;  // +inf
-        cmax[t] = 0u;
-        wmin[t] = 0x7F800000u;
-        wmax[t] = 0u;
+    int tr = (int)floor(((2.0 * T + 1.0) - sqrt((2.0 * T + 1.0) * (2.0 * T + 1.0) - 8.0 * blockIdx.x)) * 0.5);
+    while (tr > 0 && tile_row_first(tr, T) > (int)blockIdx.x) --tr;
+    while (tr + 1 < T && tile_row_first(tr + 1, T) <= (int)blockIdx.x) ++tr;
+    const int tc = tr + ((int)blockIdx.x - tile_row_first(tr, T));
+    __shared__ unsigned cmin[4][64], cmax[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = tc * 64 + lane;
+    const int i0 = tr * 64 + w * 16;
+    unsigned* out = part + (size_t)blockIdx.x * 256;
+    float d[16];
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+        const int i = i0 + rr;
+        d[rr] = (i < n && j > i && j < n) ? up[tri(n, i, j)] : 0.0f;
     }
-    __syncthreads();
-    const int r = t >> 2, c0 = (t & 3) * 16;
-    const int i = tr * 64 + r;
-    unsigned lo = 0x7F800000u, hi = 0u;
-    if (i < n) {
-        for (int c = c0; c < c0 + 16; ++c) {
-            const int j = tc * 64 + c;
-            if (j <= i || j >= n) continue;
-            const unsigned v = fbits(up[tri(n, i, j)]);
-            lo = min(lo, v);
-            hi = max(hi, v);
-            atomicMin(&cmin[c], v);
-            atomicMax(&cmax[c], v);
+    unsigned clo = 0xFFFFFFFFu, chi = 0u;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+        const int i = i0 + rr;
+        const bool ok = i < n && j > i && j < n && d[rr] == d[rr];
+        const unsigned k = fkey(fbits(d[rr]));
+        const unsigned klo = ok ? k : 0xFFFFFFFFu, khi = ok ? k : 0u;
+        clo = min(clo, klo);
+        chi = max(chi, khi);
+        const unsigned rlo = wave_min_u32(klo), rhi = wave_max_u32(khi);
+        if (lane == 0) {
+            out[w * 16 + rr] = rlo;
+            out[64 + w * 16 + rr] = rhi;
         }
     }
-    atomicMin(&wmin[r], lo);
-    atomicMax(&wmax[r], hi);
+    cmin[w][lane] = clo;
+    cmax[w][lane] = chi;
     __syncthreads();
-    if (t < 64) {
-        const int ri = tr * 64 + t, cj = tc * 64 + t;
-        if (ri < n && wmin[t] != 0x7F800000u) {
-            atomicMin(rmin + ri, wmin[t]);
-            atomicMax(rmax + ri, wmax[t]);
+    if (w == 0) {
+        out[128 + lane] = min(min(cmin[0][lane], cmin[1][lane]), min(cmin[2][lane], cmin[3][lane]));
+        out[192 + lane] = max(max(cmax[0][lane], cmax[1][lane]), max(cmax[2][lane], cmax[3][lane]));
+    }
+}
+
+// Block t: elements e = 64 t + lane.  Element e's row part is in tiles
+// (t, tc >= t), its column part in tiles (tr <= t, t); wave q of the block
+// takes every eighth of them, the eight waves are combined in LDS, and the
+// result, started from the keys of f32::MAX and -f32::MAX, is decoded to f32.
+__global__ __launch_bounds__(512) void row_stats_reduce_kernel(const unsigned* __restrict__ part, int n, int T,
+                                                               unsigned* __restrict__ rmin,
+                                                               unsigned* __restrict__ rmax)
+{
+    __shared__ unsigned slo[8][64], shi[8][64];
+    const int t = blockIdx.x, r = threadIdx.x & 63, q = threadIdx.x >> 6;
+    unsigned lo = 0xFFFFFFFFu, hi = 0u;
+    if (n >= 2) {
+        const int f = tile_row_first(t, T);
+        for (int tc = t + q; tc < T; tc += 8) {
+            const unsigned* p = part + (size_t)(f + tc - t) * 256;
+            lo = min(lo, p[r]);
+            hi = max(hi, p[64 + r]);
         }
-        if (cj < n && cmin[t] != 0x7F800000u) {
-            atomicMin(rmin + cj, cmin[t]);
-            atomicMax(rmax + cj, cmax[t]);
+        for (int tr = q; tr <= t; tr += 8) {
+            const unsigned* p = part + (size_t)(tile_row_first(tr, T) + t - tr) * 256;
+            lo = min(lo, p[128 + r]);
+            hi = max(hi, p[192 + r]);
         }
+    }
+    slo[q][r] = lo;
+    shi[q][r] = hi;
+    __syncthreads();
+    const int e = t * 64 + r;
+    if (q == 0 && e < n) {
+        lo = kKeyFltMax;
+        hi = kKeyNegFltMax;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            lo = min(lo, slo[k][r]);
+            hi = max(hi, shi[k][r]);
+        }
+        rmin[e] = fkey_inverse(lo);
+        rmax[e] = fkey_inverse(hi);
     }
 }
 
@@ -359,15 +414,19 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(const float* __res
 
 }  // namespace
 
-void launch_row_stats(const float* up, int n, float* rmin, float* rmax, hipStream_t s)
+size_t row_stats_scratch_bytes(int n)
 {
-    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256));
-    fill_u32_kernel<<<(int)nb, 256, 0, s>>>(reinterpret_cast<unsigned*>(rmin), 0x7F800000u, n);
-    fill_u32_kernel<<<(int)nb, 256, 0, s>>>(reinterpret_cast<unsigned*>(rmax), 0u, n);
+    const size_t T = (size_t)(n + 63) / 64;
+    return std::max<size_t>(1, T * (T + 1) / 2) * 256 * 4;
+}
+
+void launch_row_stats(const float* up, int n, float* rmin, float* rmax, void* scratch, hipStream_t s)
+{
     const int T = (n + 63) / 64;
-    if (n >= 2)
-        row_stats_kernel<<<dim3(T, T), 256, 0, s>>>(up, n, reinterpret_cast<unsigned*>(rmin),
-                                                   reinterpret_cast<unsigned*>(rmax));
+    auto* part = static_cast<unsigned*>(scratch);
+    if (n >= 2) row_stats_tile_kernel<<<T * (T + 1) / 2, 256, 0, s>>>(up, n, T, part);
+    row_stats_reduce_kernel<<<T, 512, 0, s>>>(part, n, T, reinterpret_cast<unsigned*>(rmin),
+                                              reinterpret_cast<unsigned*>(rmax));
 }
 
 void launch_members_row_max(const float* up, int n, const unsigned* members, int m, float* out,

@@ -112,6 +112,25 @@ def test_row_stats_outliers_medoid(gpu_ctx):
     assert lo1[0] == np.float32(np.finfo(np.float32).max) and hi1[0] == -np.float32(np.finfo(np.float32).max)
 
 
+@pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 130, 257])
+def test_row_stats_shapes_ties_and_inf(gpu_ctx, n):
+    """Row minima / maxima over tile edges, with tied values, negative values,
+    +-inf and NaN entries, against the oracle (the reference's `d < min`,
+    `d > max` scans from f32::MAX and -f32::MAX: NaN never counts, an
+    all-+inf row keeps f32::MAX as its minimum)."""
+    rng = np.random.default_rng(n)
+    up = rng.integers(-4, 6, n * (n - 1) // 2).astype(np.float32) * np.float32(0.25)
+    up[up == 0] = np.float32(0.5)  # no -0.0 / +0.0 pairs: equal values, different bits
+    r = rng.random(up.size)
+    up[r < 0.1] = np.float32(np.inf)
+    up[(r >= 0.1) & (r < 0.13)] = np.float32(np.nan)
+    up[(r >= 0.13) & (r < 0.14)] = -np.float32(np.inf)
+    m = B.RmsdMatrix(n, up)
+    lo, hi = C.row_stats(m, ctx=gpu_ctx)
+    wlo, whi = O.row_stats(up, n)
+    assert np.array_equal(lo, wlo) and np.array_equal(hi, whi)
+
+
 def test_clustering_2000_timing(gpu_ctx):
     n, L = 2000, 100
     x = B.generate_ensemble(9, n, L, sigma=(0.5, 3.0))
