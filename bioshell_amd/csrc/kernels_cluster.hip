@@ -16,6 +16,7 @@
 #include "wave.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace bs {
 namespace {
@@ -46,19 +47,21 @@ constexpr unsigned kKeyNegFltMax = 0x00800000u;  // fkey(bits of -f32::MAX)
 // triangle of tiles (tr <= tc).
 __host__ __device__ inline int tile_row_first(int r, int T) { return r * T - r * (r - 1) / 2; }
 
-// Block b is tile (tr, tc), tr <= tc: rows 64 tr.., columns 64 tc.. of the
-// packed triangle.  Wave w takes rows 64 tr + 16 w .. + 15; lane l holds
+// Block b covers tile base + b = (tr, tc), tr <= tc: rows 64 tr.., columns
+// 64 tc.. of the packed triangle.  Wave w takes rows 64 tr + 16 w .. + 15; lane l holds
 // column 64 tc + l, so every row's 64 values are one 256-byte read, and the
 // wave issues its 16 reads before using any.  The tile's results go to
 // part[256 b ..]: row minima, row maxima, column minima, column maxima (64
-// keys each; a row or column with no entry here holds the identity).
+// keys each; a row or column with no entry here holds the identity).  base
+// is the first tile of the chunk of tile rows this launch covers.
 __global__ __launch_bounds__(256) void row_stats_tile_kernel(const float* __restrict__ up, int n, int T,
-                                                             unsigned* __restrict__ part)
+                                                             int base, unsigned* __restrict__ part)
 {
-    int tr = (int)floor(((2.0 * T + 1.0) - sqrt((2.0 * T + 1.0) * (2.0 * T + 1.0) - 8.0 * blockIdx.x)) * 0.5);
-    while (tr > 0 && tile_row_first(tr, T) > (int)blockIdx.x) --tr;
-    while (tr + 1 < T && tile_row_first(tr + 1, T) <= (int)blockIdx.x) ++tr;
-    const int tc = tr + ((int)blockIdx.x - tile_row_first(tr, T));
+    const int b = base + (int)blockIdx.x;  // tile index in the whole triangle of tiles
+    int tr = (int)floor(((2.0 * T + 1.0) - sqrt((2.0 * T + 1.0) * (2.0 * T + 1.0) - 8.0 * b)) * 0.5);
+    while (tr > 0 && tile_row_first(tr, T) > b) --tr;
+    while (tr + 1 < T && tile_row_first(tr + 1, T) <= b) ++tr;
+    const int tc = tr + (b - tile_row_first(tr, T));
     __shared__ unsigned cmin[4][64], cmax[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int j = tc * 64 + lane;
@@ -94,26 +97,32 @@ __global__ __launch_bounds__(256) void row_stats_tile_kernel(const float* __rest
     }
 }
 
-// Block t: elements e = 64 t + lane.  Element e's row part is in tiles
-// (t, tc >= t), its column part in tiles (tr <= t, t); wave q of the block
-// takes every eighth of them, the eight waves are combined in LDS, and the
-// result, started from the keys of f32::MAX and -f32::MAX, is decoded to f32.
+// Block t: elements e = 64 t + lane, for the chunk of tile rows [t0, t1)
+// whose tile results are in part (tile index minus base).  Element e's row
+// part is in tiles (t, tc >= t), its column part in tiles (tr <= t, t); wave q
+// takes every eighth of those in the chunk, and the eight waves are combined
+// in LDS with the element's keys so far: the keys of f32::MAX and -f32::MAX
+// for the first chunk, else the values rmin / rmax hold.  The last chunk
+// decodes the keys to f32.
 __global__ __launch_bounds__(512) void row_stats_reduce_kernel(const unsigned* __restrict__ part, int n, int T,
+                                                               int t0, int t1, int base, int first, int last,
                                                                unsigned* __restrict__ rmin,
                                                                unsigned* __restrict__ rmax)
 {
     __shared__ unsigned slo[8][64], shi[8][64];
     const int t = blockIdx.x, r = threadIdx.x & 63, q = threadIdx.x >> 6;
     unsigned lo = 0xFFFFFFFFu, hi = 0u;
-    if (n >= 2) {
-        const int f = tile_row_first(t, T);
-        for (int tc = t + q; tc < T; tc += 8) {
-            const unsigned* p = part + (size_t)(f + tc - t) * 256;
-            lo = min(lo, p[r]);
-            hi = max(hi, p[64 + r]);
+    if (n >= 2 && t >= t0) {
+        if (t < t1) {
+            const int f = tile_row_first(t, T) - base;
+            for (int tc = t + q; tc < T; tc += 8) {
+                const unsigned* p = part + (size_t)(f + tc - t) * 256;
+                lo = min(lo, p[r]);
+                hi = max(hi, p[64 + r]);
+            }
         }
-        for (int tr = q; tr <= t; tr += 8) {
-            const unsigned* p = part + (size_t)(tile_row_first(tr, T) + t - tr) * 256;
+        for (int tr = t0 + q; tr < t1 && tr <= t; tr += 8) {
+            const unsigned* p = part + (size_t)(tile_row_first(tr, T) - base + t - tr) * 256;
             lo = min(lo, p[128 + r]);
             hi = max(hi, p[192 + r]);
         }
@@ -123,15 +132,15 @@ __global__ __launch_bounds__(512) void row_stats_reduce_kernel(const unsigned* _
     __syncthreads();
     const int e = t * 64 + r;
     if (q == 0 && e < n) {
-        lo = kKeyFltMax;
-        hi = kKeyNegFltMax;
+        lo = first ? kKeyFltMax : rmin[e];
+        hi = first ? kKeyNegFltMax : rmax[e];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             lo = min(lo, slo[k][r]);
             hi = max(hi, shi[k][r]);
         }
-        rmin[e] = fkey_inverse(lo);
-        rmax[e] = fkey_inverse(hi);
+        rmin[e] = last ? fkey_inverse(lo) : lo;
+        rmax[e] = last ? fkey_inverse(hi) : hi;
     }
 }
 
@@ -414,19 +423,45 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(const float* __res
 
 }  // namespace
 
+// Tiles per launch: the scratch holds the results (1 KiB each) of at most
+// this many tiles, or of one tile row if that is longer, so it stays bounded
+// for any n.  BIOSHELL_ROW_STATS_CHUNK_TILES lowers it (tests of the chunking).
+static int row_stats_chunk_tiles()
+{
+    int c = 1 << 16;
+    if (const char* e = std::getenv("BIOSHELL_ROW_STATS_CHUNK_TILES")) c = std::max(1, std::atoi(e));
+    return c;
+}
+
 size_t row_stats_scratch_bytes(int n)
 {
     const size_t T = (size_t)(n + 63) / 64;
-    return std::max<size_t>(1, T * (T + 1) / 2) * 256 * 4;
+    const size_t cap = std::max<size_t>((size_t)row_stats_chunk_tiles(), T);
+    return std::max<size_t>(1, std::min<size_t>(T * (T + 1) / 2, cap)) * 256 * 4;
 }
 
 void launch_row_stats(const float* up, int n, float* rmin, float* rmax, void* scratch, hipStream_t s)
 {
     const int T = (n + 63) / 64;
     auto* part = static_cast<unsigned*>(scratch);
-    if (n >= 2) row_stats_tile_kernel<<<T * (T + 1) / 2, 256, 0, s>>>(up, n, T, part);
-    row_stats_reduce_kernel<<<T, 512, 0, s>>>(part, n, T, reinterpret_cast<unsigned*>(rmin),
-                                              reinterpret_cast<unsigned*>(rmax));
+    auto* a = reinterpret_cast<unsigned*>(rmin);
+    auto* b = reinterpret_cast<unsigned*>(rmax);
+    if (n < 2) {
+        row_stats_reduce_kernel<<<T, 512, 0, s>>>(part, n, T, 0, 0, 0, 1, 1, a, b);
+        return;
+    }
+    // chunks of whole tile rows, each within the chunk size (or one tile row)
+    const int chunk = row_stats_chunk_tiles();
+    int t0 = 0;
+    while (t0 < T) {
+        const int base = t0 * T - t0 * (t0 - 1) / 2;
+        int t1 = t0 + 1;
+        while (t1 < T && (t1 + 1) * T - (t1 + 1) * t1 / 2 - base <= chunk) ++t1;
+        const int tiles = t1 * T - t1 * (t1 - 1) / 2 - base;
+        row_stats_tile_kernel<<<tiles, 256, 0, s>>>(up, n, T, base, part);
+        row_stats_reduce_kernel<<<T, 512, 0, s>>>(part, n, T, t0, t1, base, t0 == 0, t1 == T, a, b);
+        t0 = t1;
+    }
 }
 
 void launch_members_row_max(const float* up, int n, const unsigned* members, int m, float* out,

@@ -131,6 +131,20 @@ def test_row_stats_shapes_ties_and_inf(gpu_ctx, n):
     assert np.array_equal(lo, wlo) and np.array_equal(hi, whi)
 
 
+@pytest.mark.parametrize("chunk", ["1", "7", "40"])
+def test_row_stats_in_chunks_of_tile_rows(gpu_ctx, monkeypatch, chunk):
+    """Row statistics computed over chunks of tile rows (the bounded-scratch
+    path large n takes) equal the oracle's."""
+    monkeypatch.setenv("BIOSHELL_ROW_STATS_CHUNK_TILES", chunk)
+    n = 700
+    rng = np.random.default_rng(int(chunk))
+    up = rng.integers(-3, 40, n * (n - 1) // 2).astype(np.float32) * np.float32(0.125)
+    up[rng.random(up.size) < 0.05] = np.float32(np.inf)
+    lo, hi = C.row_stats(B.RmsdMatrix(n, up), ctx=gpu_ctx)
+    wlo, whi = O.row_stats(up, n)
+    assert np.array_equal(lo, wlo) and np.array_equal(hi, whi)
+
+
 def test_clustering_2000_timing(gpu_ctx):
     n, L = 2000, 100
     x = B.generate_ensemble(9, n, L, sigma=(0.5, 3.0))
