@@ -38,14 +38,13 @@ void cluster_device(bs_context* ctx, const float* d_upper, size_t n, int rule, u
     const size_t ld = (n + 63) / 64 * 64;  // 256-byte rows: the scan reads 16-byte groups
     float* dm = static_cast<float*>(D.ensure(n * ld * 4, "clustering matrix"));
     const size_t cand = cluster_candidate_bytes();
-    char* a = static_cast<char*>(aux.ensure(n * 4 * 4 + 256 + ld * 4 + 256 + (size_t)n_part * cand,
+    char* a = static_cast<char*>(aux.ensure(n * 4 * 4 + 256 + 256 + (size_t)n_part * cand,
                                             "clustering state"));
     unsigned* sizes = reinterpret_cast<unsigned*>(a);
     unsigned* mi = reinterpret_cast<unsigned*>(a + n * 4);
     unsigned* mj = reinterpret_cast<unsigned*>(a + n * 8);
     float* md = reinterpret_cast<float*>(a + n * 12);
-    float* rowj = reinterpret_cast<float*>(a + (n * 16 + 255) / 256 * 256);  // 16-byte aligned
-    void* state = reinterpret_cast<char*>(rowj) + ld * 4;
+    void* state = a + (n * 16 + 255) / 256 * 256;
     void* part = static_cast<char*>(state) + 256;
     BS_REQUIRE(cluster_state_bytes() <= 256, "merge state too large");
     launch_cluster_init(d_upper, N, (int)ld, dm, sizes, state, s);
@@ -69,7 +68,7 @@ void cluster_device(bs_context* ctx, const float* d_upper, size_t n, int rule, u
     BS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
         for (int k = 0; k < per_graph; ++k)
-            launch_cluster_step(dm, N, (int)ld, state, rowj, part, n_part, sizes, rule, mi, mj, md, s);
+            launch_cluster_step(dm, N, (int)ld, state, part, n_part, sizes, rule, mi, mj, md, s);
     } catch (...) {
         (void)hipStreamEndCapture(s, &graph);
         if (graph) (void)hipGraphDestroy(graph);

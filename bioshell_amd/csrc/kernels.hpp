@@ -174,16 +174,15 @@ void launch_members_row_max(const float* up, int n, const unsigned* members, int
 // Agglomerative clustering: the dense matrix D (n rows of ld >= n floats, ld a
 // multiple of 4), sizes and a cleared merge state from the packed triangle;
 // then n - 1 merge steps (n_part work-groups scan, one decides; state holds
-// cluster_state_bytes(), rowj ld floats, part n_part candidates of
-// cluster_candidate_bytes()).
+// cluster_state_bytes(), part n_part candidates of cluster_candidate_bytes()).
 size_t cluster_state_bytes();
 size_t cluster_candidate_bytes();
 void launch_cluster_init(const float* up, int n, int ld, float* D, unsigned* sizes, void* state, hipStream_t s);
 // One merge; the merge index lives in `state`, so every step's launches are
 // identical (a captured run of steps replays as is; steps past the last merge
 // do nothing).
-void launch_cluster_step(float* D, int n, int ld, void* state, float* rowj, void* part, int n_part,
-                         unsigned* sizes, int rule, unsigned* mi, unsigned* mj, float* md, hipStream_t s);
+void launch_cluster_step(float* D, int n, int ld, void* state, void* part, int n_part, unsigned* sizes,
+                         int rule, unsigned* mi, unsigned* mj, float* md, hipStream_t s);
 
 // Single-pair superposition (rotation + translation) in FP64.
 void launch_superpose(const double* a, const double* b, int L, double* out13, hipStream_t s);

@@ -232,9 +232,10 @@ struct MergeState {
 };
 
 // Merges are applied and found by two launches per merge:
-// cluster_step_kernel (every work-group) applies the previous merge, decided
-// by the previous cluster_decide_kernel, and finds each work-group's best
-// candidate; cluster_decide_kernel (one work-group) takes the best of them.
+// cluster_decide_kernel (one work-group) takes the best of the work-groups'
+// candidates for merge s and applies it to rows i and j; the next
+// cluster_step_kernel (every work-group) applies it to every other row and
+// finds each work-group's best candidate for merge s + 1.
 //   update_distances(i, j, rule, i): res[k] = rule(size_i, size_j, sizes[j],
 //     d_ij, D[i][k], D[j][k]), written to row and column i, D[i][i] = 0
 //     (clustering_matrix.rs:57-74);
@@ -243,17 +244,17 @@ struct MergeState {
 //   closest pair: the first strict minimum of the scan j ascending, i
 //     ascending (clustering_matrix.rs:27-42), i.e. the smallest (value, j, i)
 //     over values below f32::MAX, else (0, 0).
-// Each wave owns rows r = W k + wave id: it applies the merge to its own rows
-// and then scans their prefixes D[r][0..r) (= column r above the diagonal),
-// four floats per load.  By symmetry the update of row r needs only its own
-// old D[r][i], D[r][j], D[r][last]; the owner of row i also needs the old row
-// j, which the owner of row j overwrites meanwhile -- the decide kernel keeps a
-// copy of it.  Nobody writes the last row or column (it leaves the active
-// part), so they are read in place.  No wave reads another's rows otherwise,
-// so the update and the scan need no barrier between work-groups.
+// Each wave of the step kernel owns rows r = W k + wave id.  In a row other
+// than i and j the merge changes two entries only, D[r][i] and D[r][j], and by
+// symmetry their new values need only the row's own old D[r][i], D[r][j] and
+// D[r][last]; the wave scans the row's prefix D[r][0..r) (= column r above the
+// diagonal) four floats per load with those two columns skipped, takes their
+// new values as candidates, and stores them afterwards.  No wave reads
+// another's rows, so the kernel needs no barrier between work-groups, and no
+// store has to land before a scan.  Rows i and j, rewritten whole by the
+// decide kernel, are scanned as they are.
 __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D, int n, int ld,
                                                            const MergeState* __restrict__ st,
-                                                           const float* __restrict__ rowj,
                                                            Cand* __restrict__ part, int rule)
 {
     __shared__ Cand wbest[4];
@@ -262,7 +263,6 @@ __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int W = gridDim.x * 4, wid = blockIdx.x * 4 + w;
     const int order = n - s;  // active elements after the previous merge
-    const float* rlast = D + (int64_t)order * ld;  // the previous last row (index `order`)
     int mi_ = -1, mj_ = -1;
     bool moved = false;
     unsigned msi = 0, msj = 0;
@@ -275,82 +275,99 @@ __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D
         msj = st->sj;
         mdij = st->dij;
     }
-    Cand best{3.402823466e+38f, 0x7FFFFFFF, 0x7FFFFFFF};
+    const float kFltMax = 3.402823466e+38f, kInf = __builtin_inff();
+    Cand best{kFltMax, 0x7FFFFFFF, 0x7FFFFFFF};
     for (int r = wid; r < order; r += W) {
         float* row = D + (int64_t)r * ld;
-        if (s > 0) {
-            // res[last] (the merged cluster to the element that moves to j)
-            // comes from column `order` of rows i and j, which nobody writes
-            if (moved && r == mj_) {  // first: with the (0, 0) pair row i is row j
-                const float rl = merge_rule(rule, msi, msj, msj, mdij, D[(int64_t)mi_ * ld + order],
-                                            D[(int64_t)mj_ * ld + order]);
-                for (int c = lane; c < order; c += 64) row[c] = c == mj_ ? 0.0f : (c == mi_ ? rl : rlast[c]);
-            } else if (r == mi_) {
-                const float rl = moved ? merge_rule(rule, msi, msj, msj, mdij, row[order], rowj[order]) : 0.0f;
-                for (int c = lane; c < order; c += 64) {
-                    float v;
-                    if (c == mi_)
-                        v = 0.0f;
-                    else if (moved && c == mj_)
-                        v = rl;
-                    else
-                        v = merge_rule(rule, msi, msj, msj, mdij, row[c], rowj[c]);
-                    row[c] = v;
+        const bool patch = s > 0 && r != mi_ && !(moved && r == mj_);
+        int pc0 = -1, pc1 = -1;  // patched columns
+        float pv0 = 0.0f, pv1 = 0.0f;
+        if (patch) {
+            const float vi = row[mi_], vj = row[mj_], vl = row[order];
+            pc0 = mi_;
+            if (moved && mj_ == mi_) {  // the (0, 0) pair: the move wins
+                pv0 = vl;
+            } else {
+                pv0 = merge_rule(rule, msi, msj, msj, mdij, vi, vj);
+                if (moved) {
+                    pc1 = mj_;
+                    pv1 = vl;
                 }
-            } else if (lane == 0) {
-                const float vi = row[mi_], vj = row[mj_], vl = row[order];
-                row[mi_] = merge_rule(rule, msi, msj, msj, mdij, vi, vj);
-                if (moved) row[mj_] = vl;
             }
-            // the lanes' writes to this row are read back by the scan below
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (r == 0) continue;
-        // smallest value of the prefix, first index on ties: lane l takes the
-        // 4-float groups l, l + 64, ... (rows are 16-byte aligned, ld % 4 == 0)
-        float v = 3.402823466e+38f;
-        int bi = 0x7FFFFFFF;
-        const int r4 = r >> 2;
-        const float4* row4 = reinterpret_cast<const float4*>(row);
-        int g = lane;
-        for (; g + 192 < r4; g += 256) {
-            const float4 x0 = row4[g], x1 = row4[g + 64], x2 = row4[g + 128], x3 = row4[g + 192];
-            const float4 xs[4] = {x0, x1, x2, x3};
+        if (r > 0) {
+            // smallest value of the prefix, first index on ties: lane l takes the
+            // 4-float groups l, l + 64, ... (rows are 16-byte aligned, ld % 4 == 0);
+            // the patched columns read as +inf (never a candidate)
+            auto mask = [&](float4 x, int g4) {
+                if (pc0 >> 2 == g4) {
+                    const int k = pc0 & 3;
+                    x.x = k == 0 ? kInf : x.x; x.y = k == 1 ? kInf : x.y;
+                    x.z = k == 2 ? kInf : x.z; x.w = k == 3 ? kInf : x.w;
+                }
+                if (pc1 >> 2 == g4) {
+                    const int k = pc1 & 3;
+                    x.x = k == 0 ? kInf : x.x; x.y = k == 1 ? kInf : x.y;
+                    x.z = k == 2 ? kInf : x.z; x.w = k == 3 ? kInf : x.w;
+                }
+                return x;
+            };
+            float v = kFltMax;
+            int bi = 0x7FFFFFFF;
+            const int r4 = r >> 2;
+            const float4* row4 = reinterpret_cast<const float4*>(row);
+            int g = lane;
+            for (; g + 192 < r4; g += 256) {
+                const float4 x0 = row4[g], x1 = row4[g + 64], x2 = row4[g + 128], x3 = row4[g + 192];
+                const float4 xs[4] = {mask(x0, g), mask(x1, g + 64), mask(x2, g + 128), mask(x3, g + 192)};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int c0 = 4 * (g + 64 * u);
-                if (xs[u].x < v) { v = xs[u].x; bi = c0; }
-                if (xs[u].y < v) { v = xs[u].y; bi = c0 + 1; }
-                if (xs[u].z < v) { v = xs[u].z; bi = c0 + 2; }
-                if (xs[u].w < v) { v = xs[u].w; bi = c0 + 3; }
+                for (int u = 0; u < 4; ++u) {
+                    const int c0 = 4 * (g + 64 * u);
+                    if (xs[u].x < v) { v = xs[u].x; bi = c0; }
+                    if (xs[u].y < v) { v = xs[u].y; bi = c0 + 1; }
+                    if (xs[u].z < v) { v = xs[u].z; bi = c0 + 2; }
+                    if (xs[u].w < v) { v = xs[u].w; bi = c0 + 3; }
+                }
             }
-        }
-        for (; g < r4; g += 64) {
-            const float4 x = row4[g];
-            const int c0 = 4 * g;
-            if (x.x < v) { v = x.x; bi = c0; }
-            if (x.y < v) { v = x.y; bi = c0 + 1; }
-            if (x.z < v) { v = x.z; bi = c0 + 2; }
-            if (x.w < v) { v = x.w; bi = c0 + 3; }
-        }
-        const int c_tail = 4 * r4 + lane;
-        if (c_tail < r) {
-            const float x = row[c_tail];
-            if (x < v) { v = x; bi = c_tail; }
-        }
+            for (; g < r4; g += 64) {
+                const float4 x = mask(row4[g], g);
+                const int c0 = 4 * g;
+                if (x.x < v) { v = x.x; bi = c0; }
+                if (x.y < v) { v = x.y; bi = c0 + 1; }
+                if (x.z < v) { v = x.z; bi = c0 + 2; }
+                if (x.w < v) { v = x.w; bi = c0 + 3; }
+            }
+            const int c_tail = 4 * r4 + lane;
+            if (c_tail < r && c_tail != pc0 && c_tail != pc1) {
+                const float x = row[c_tail];
+                if (x < v) { v = x; bi = c_tail; }
+            }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const float ov = __shfl_xor(v, o);
-            const int ob = __shfl_xor(bi, o);
-            if (ov < v || (ov == v && ob < bi)) {
-                v = ov;
-                bi = ob;
+            for (int o = 32; o > 0; o >>= 1) {
+                const float ov = __shfl_xor(v, o);
+                const int ob = __shfl_xor(bi, o);
+                if (ov < v || (ov == v && ob < bi)) {
+                    v = ov;
+                    bi = ob;
+                }
             }
+            // the patched prefix entries, under the scan's rule: a value below
+            // f32::MAX, first index on ties
+            if (pc0 >= 0 && pc0 < r && pv0 < kFltMax && (pv0 < v || (pv0 == v && pc0 < bi))) {
+                v = pv0;
+                bi = pc0;
+            }
+            if (pc1 >= 0 && pc1 < r && pv1 < kFltMax && (pv1 < v || (pv1 == v && pc1 < bi))) {
+                v = pv1;
+                bi = pc1;
+            }
+            const Cand c{v, r, bi};
+            if (bi != 0x7FFFFFFF && better(c, best)) best = c;
         }
-        const Cand c{v, r, bi};
-        if (bi != 0x7FFFFFFF && better(c, best)) best = c;
+        if (patch && lane == 0) {
+            row[pc0] = pv0;
+            if (pc1 >= 0) row[pc1] = pv1;
+        }
     }
     if (lane == 0) wbest[w] = best;
     __syncthreads();
@@ -362,23 +379,25 @@ __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D
     }
 }
 
-// The closest pair over the work-groups' candidates: records the merge, sets
-// sizes[i] = size_j + size_i (hierarchical.rs:66) and copies the old row j for
-// the next update.
-__global__ __launch_bounds__(1024) void cluster_decide_kernel(const float* __restrict__ D, int n, int ld,
+// The closest pair over the work-groups' candidates (merge s): records it,
+// sets sizes[i] = size_j + size_i (hierarchical.rs:66), and rewrites rows i
+// and j for the active elements c < order - 1 that remain.  Thread c reads
+// column c of the old rows i, j and last and writes column c of the new rows
+// i and j, so the threads never touch each other's data.
+__global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict__ D, int n, int ld,
                                                               MergeState* __restrict__ st,
-                                                              float* __restrict__ rowj,
                                                               const Cand* __restrict__ part, int n_part,
-                                                              unsigned* __restrict__ sizes,
+                                                              unsigned* __restrict__ sizes, int rule,
                                                               unsigned* __restrict__ mi,
                                                               unsigned* __restrict__ mj,
                                                               float* __restrict__ md)
 {
     __shared__ Cand red[1024];
     __shared__ int s_i, s_j;
+    __shared__ unsigned s_si, s_sj;
+    __shared__ float s_dij;
     const int s = st->s;  // read by every thread before thread 0 advances it below
     if (s >= n - 1) return;
-    const int order = n - s;
     Cand b{3.402823466e+38f, 0x7FFFFFFF, 0x7FFFFFFF};
     for (int k = threadIdx.x; k < n_part; k += 1024) {
         const Cand c = part[k];
@@ -396,28 +415,46 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(const float* __res
             c.i = 0;
             c.j = 0;
         }
+        const unsigned si = sizes[c.i], sj = sizes[c.j];
+        const float dij = D[(int64_t)c.i * ld + c.j];
         s_i = c.i;
         s_j = c.j;
-    }
-    __syncthreads();
-    const int i = s_i, j = s_j;
-    // old row j, columns 0 .. order - 1 (the next update reads up to index order - 1)
-    const float4* src = reinterpret_cast<const float4*>(D + (int64_t)j * ld);
-    float4* dst = reinterpret_cast<float4*>(rowj);
-    for (int k = threadIdx.x; k < (order + 3) / 4; k += 1024) dst[k] = src[k];
-    if (threadIdx.x == 0) {
-        const unsigned si = sizes[i], sj = sizes[j];
-        const float dij = D[(int64_t)i * ld + j];
-        st->i = i;
-        st->j = j;
+        s_si = si;
+        s_sj = sj;
+        s_dij = dij;
+        st->i = c.i;
+        st->j = c.j;
         st->si = si;
         st->sj = sj;
         st->dij = dij;
-        sizes[i] = sj + si;
-        mi[s] = (unsigned)i;
-        mj[s] = (unsigned)j;
+        sizes[c.i] = sj + si;
+        mi[s] = (unsigned)c.i;
+        mj[s] = (unsigned)c.j;
         md[s] = dij;
         st->s = s + 1;
+    }
+    __syncthreads();  // every read of the old rows below follows thread 0's read of D[i][j]
+    const int i = s_i, j = s_j;
+    const int order = n - s - 1;  // active elements after this merge; index `order` is the last
+    const bool moved = j < order;
+    const unsigned si = s_si, sj = s_sj;
+    const float dij = s_dij;
+    float* Ri = D + (int64_t)i * ld;
+    float* Rj = D + (int64_t)j * ld;
+    const float* Rl = D + (int64_t)order * ld;
+    // res[last]: the merged cluster to the element that moves into j
+    const float rl = moved ? merge_rule(rule, si, sj, sj, dij, Ri[order], Rj[order]) : 0.0f;
+    for (int c = threadIdx.x; c < order; c += 1024) {
+        const float oi = Ri[c], oj = Rj[c], ol = Rl[c];
+        float ni;
+        if (c == i)
+            ni = 0.0f;
+        else if (moved && c == j)
+            ni = rl;
+        else
+            ni = merge_rule(rule, si, sj, sj, dij, oi, oj);
+        Ri[c] = ni;
+        if (moved) Rj[c] = c == j ? 0.0f : (c == i ? rl : ol);  // with the (0, 0) pair the move wins
     }
 }
 
@@ -478,13 +515,13 @@ void launch_cluster_init(const float* up, int n, int ld, float* D, unsigned* siz
                                       (int64_t)(sizeof(MergeState) / sizeof(unsigned)));
 }
 
-void launch_cluster_step(float* D, int n, int ld, void* state, float* rowj, void* part, int n_part,
-                         unsigned* sizes, int rule, unsigned* mi, unsigned* mj, float* md, hipStream_t s)
+void launch_cluster_step(float* D, int n, int ld, void* state, void* part, int n_part, unsigned* sizes,
+                         int rule, unsigned* mi, unsigned* mj, float* md, hipStream_t s)
 {
     auto* st = static_cast<MergeState*>(state);
     auto* cand = static_cast<Cand*>(part);
-    cluster_step_kernel<<<n_part, 256, 0, s>>>(D, n, ld, st, rowj, cand, rule);
-    cluster_decide_kernel<<<1, 1024, 0, s>>>(D, n, ld, st, rowj, cand, n_part, sizes, mi, mj, md);
+    cluster_step_kernel<<<n_part, 256, 0, s>>>(D, n, ld, st, cand, rule);
+    cluster_decide_kernel<<<1, 1024, 0, s>>>(D, n, ld, st, cand, n_part, sizes, rule, mi, mj, md);
 }
 
 size_t cluster_state_bytes() { return sizeof(MergeState); }
