@@ -444,17 +444,51 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict_
     const float* Rl = D + (int64_t)order * ld;
     // res[last]: the merged cluster to the element that moves into j
     const float rl = moved ? merge_rule(rule, si, sj, sj, dij, Ri[order], Rj[order]) : 0.0f;
-    for (int c = threadIdx.x; c < order; c += 1024) {
-        const float oi = Ri[c], oj = Rj[c], ol = Rl[c];
-        float ni;
-        if (c == i)
-            ni = 0.0f;
-        else if (moved && c == j)
-            ni = rl;
-        else
-            ni = merge_rule(rule, si, sj, sj, dij, oi, oj);
-        Ri[c] = ni;
-        if (moved) Rj[c] = c == j ? 0.0f : (c == i ? rl : ol);  // with the (0, 0) pair the move wins
+    // four columns per thread and group, two groups per trip with every load
+    // issued before any store (rows are 16-byte aligned; columns at or past
+    // `order` keep their values)
+    float4* Ri4 = reinterpret_cast<float4*>(Ri);
+    float4* Rj4 = reinterpret_cast<float4*>(Rj);
+    const float4* Rl4 = reinterpret_cast<const float4*>(Rl);
+    const int g_end = (order + 3) >> 2;
+    for (int g0 = threadIdx.x; g0 < g_end; g0 += 2048) {
+        float4 a[2], b4[2], l4[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int g = g0 + 1024 * u;
+            if (g < g_end) {
+                a[u] = Ri4[g];
+                b4[u] = Rj4[g];
+                l4[u] = Rl4[g];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int g = g0 + 1024 * u;
+            if (g >= g_end) continue;
+            const float oi[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+            const float oj[4] = {b4[u].x, b4[u].y, b4[u].z, b4[u].w};
+            const float ol[4] = {l4[u].x, l4[u].y, l4[u].z, l4[u].w};
+            float ni[4], nj[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = 4 * g + k;
+                if (c >= order) {
+                    ni[k] = oi[k];
+                    nj[k] = oj[k];
+                    continue;
+                }
+                if (c == i)
+                    ni[k] = 0.0f;
+                else if (moved && c == j)
+                    ni[k] = rl;
+                else
+                    ni[k] = merge_rule(rule, si, sj, sj, dij, oi[k], oj[k]);
+                nj[k] = c == j ? 0.0f : (c == i ? rl : ol[k]);
+            }
+            Ri4[g] = make_float4(ni[0], ni[1], ni[2], ni[3]);
+            if (moved) Rj4[g] = make_float4(nj[0], nj[1], nj[2], nj[3]);  // with the (0, 0) pair the move wins
+        }
     }
 }
 
