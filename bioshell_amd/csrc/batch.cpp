@@ -83,9 +83,9 @@ struct Plan {
     bool gram = true;
     GramMode mode = kGramBF16x3;
     std::vector<int> src;  // caller's index of each planned ensemble (n >= 2)
-    std::vector<int> n, L, n_pad, n_kblk, tile, slices;
+    std::vector<int> n, L, n_pad, n_kblk, tile, n_ref, slices;
     std::vector<int64_t> xyz_off, ref_off, part_off, pan_off, con_off, out_off;
-    std::vector<int> conf_prefix, pad_prefix, slice_prefix, coord_prefix;
+    std::vector<int> conf_prefix, samp_prefix, pad_prefix, slice_prefix, coord_prefix;
     std::vector<int64_t> word_prefix;
     std::vector<uint32_t> items;
     int list_start[kXcdLists + 1] = {};
@@ -107,23 +107,26 @@ Plan make_plan(const bs_context* ctx, bs_method m, const size_t* n_conf, const s
         if (n_conf[e] >= 2) P.src.push_back((int)e);
     const int E = (int)P.src.size();
     P.conf_prefix.assign(E + 1, 0);
+    P.samp_prefix.assign(E + 1, 0);
     P.pad_prefix.assign(E + 1, 0);
     P.slice_prefix.assign(E + 1, 0);
     P.coord_prefix.assign(E + 1, 0);
     P.word_prefix.assign(E + 1, 0);
-    int64_t conf_total = 0, pad_total = 0, slice_total = 0, coord_total = 0;
+    int64_t conf_total = 0, samp_total = 0, pad_total = 0, slice_total = 0, coord_total = 0;
     std::vector<double> cost(E, 0.0);
     for (int e = 0; e < E; ++e) {
         const int n = (int)n_conf[P.src[e]], L = (int)n_atoms[P.src[e]];
         const int tile = force_tile ? force_tile : choose_tile(ctx, m, n, L);
         const int n_pad = round_up(n, tile), n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
-        const int slices = (n + kMeanSliceConfs - 1) / kMeanSliceConfs;
+        const int n_ref = ref_sample_count(n);
+        const int slices = (n_ref + kMeanSliceConfs - 1) / kMeanSliceConfs;
         const int64_t pairs = (int64_t)n * (n - 1) / 2;
         P.n.push_back(n);
         P.L.push_back(L);
         P.n_pad.push_back(n_pad);
         P.n_kblk.push_back(n_kblk);
         P.tile.push_back(tile);
+        P.n_ref.push_back(n_ref);
         P.slices.push_back(slices);
         P.th.push_back(gram_thresholds(P.mode, L, n_kblk));
         P.xyz_off.push_back(P.xyz_doubles);
@@ -140,12 +143,14 @@ Plan make_plan(const bs_context* ctx, bs_method m, const size_t* n_conf, const s
         P.out_values += pairs;
         P.pairs += (uint64_t)pairs;
         conf_total += n;
+        samp_total += n_ref;
         pad_total += n_pad;
         slice_total += slices;
         coord_total += 3 * L;
         BS_REQUIRE(conf_total < (1ll << 30) && pad_total < (1ll << 30) && coord_total < (1ll << 30),
                    "batch too large");
         P.conf_prefix[e + 1] = (int)conf_total;
+        P.samp_prefix[e + 1] = (int)samp_total;
         P.pad_prefix[e + 1] = (int)pad_total;
         P.slice_prefix[e + 1] = (int)slice_total;
         P.coord_prefix[e + 1] = (int)coord_total;
@@ -283,6 +288,7 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
         b.L = P.L[e];
         b.n_pad = P.n_pad[e];
         b.n_kblk = P.n_kblk[e];
+        b.n_ref = P.n_ref[e];
         b.n_slices = P.slices[e];
         b.tile = P.tile[e];
         b.acc_coef = P.th[e].acc_coef;
@@ -291,7 +297,7 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     std::vector<unsigned char> blob;
     blob.reserve(4096);
     const size_t o_ens = put(blob, ens.data(), ens.size() * sizeof(BatchEns));
-    const size_t o_cp = put(blob, P.conf_prefix.data(), P.conf_prefix.size() * 4);
+    const size_t o_cp = put(blob, P.samp_prefix.data(), P.samp_prefix.size() * 4);
     const size_t o_pp = put(blob, P.pad_prefix.data(), P.pad_prefix.size() * 4);
     const size_t o_sp = put(blob, P.slice_prefix.data(), P.slice_prefix.size() * 4);
     const size_t o_kp = put(blob, P.coord_prefix.data(), P.coord_prefix.size() * 4);
@@ -320,12 +326,12 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     a.ens = reinterpret_cast<const BatchEns*>(meta + o_ens);
     a.n_ens = E;
     a.mode = P.mode;
-    a.conf_prefix = reinterpret_cast<const int*>(meta + o_cp);
+    a.samp_prefix = reinterpret_cast<const int*>(meta + o_cp);
     a.pad_prefix = reinterpret_cast<const int*>(meta + o_pp);
     a.slice_prefix = reinterpret_cast<const int*>(meta + o_sp);
     a.coord_prefix = reinterpret_cast<const int*>(meta + o_kp);
     a.word_prefix = reinterpret_cast<const int64_t*>(meta + o_wp);
-    a.total_conf = (int)total_conf;
+    a.total_samp = P.samp_prefix[E];
     a.total_pad = P.pad_prefix[E];
     a.total_slices = P.slice_prefix[E];
     a.total_coords = P.coord_prefix[E];

@@ -17,7 +17,7 @@ size_t prepared_bytes(int n, int L)
 {
     const size_t n_pad = (size_t)round_up(n, kTileRows);
     const size_t n_kblk = (size_t)(L + kAtomBlock - 1) / kAtomBlock;
-    const size_t slices = (size_t)(n + kMeanSliceConfs - 1) / kMeanSliceConfs;
+    const size_t slices = (size_t)(ref_sample_count(n) + kMeanSliceConfs - 1) / kMeanSliceConfs;
     return (size_t)n * 3 * 8 + (size_t)n * 9 * 8 + (size_t)(3 * L + 6) * 8 +
            slices * 3 * L * 8 + n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16 +
            (size_t)kNumConsts * n_pad * 8;
@@ -42,7 +42,7 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
     if (fixed_ref) {
         BS_HIP(hipMemcpyAsync(ref, fixed_ref, (size_t)(3 * L + 6) * 8, hipMemcpyDeviceToDevice, s));
     } else {
-        const size_t slices = (size_t)(n + kMeanSliceConfs - 1) / kMeanSliceConfs;
+        const size_t slices = (size_t)(ref_sample_count(n) + kMeanSliceConfs - 1) / kMeanSliceConfs;
         double* partial = static_cast<double*>(
             P.partial.ensure(slices * 3 * L * 8, "reference partial sums"));
         launch_reference(d_xyz, n, L, rot, cent, partial, ref, s);

@@ -34,11 +34,33 @@ constexpr int kPanelBlockU4 = 384;  // 16-byte units per (conf block, atom block
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-constexpr int kMeanSliceConfs = 32;  // conformations per partial sum of the reference mean
+constexpr int kMeanSliceConfs = 8;  // sampled conformations per partial sum of the reference mean
 
-// Reference of one ensemble (prep_device.hpp): centroids (cent, n x 3) and
-// rotations onto conformation 0 (rot, n x 9) as by-products, the slices'
-// partial sums (partial, ceil(n/32) x 3L), then r (ref: 3L values) and R0 (6).
+// The reference r of the shifted frame is the Procrustes mean of S(n)
+// conformations spread evenly over the ensemble: sample s is conformation
+// floor(s n / S), so S = n takes every one.  Any r makes the frame exact; r
+// only sets the residual sizes, hence the split's error and the flags.  A
+// mean over S samples departs from the full mean by about sigma / sqrt(S)
+// per coordinate, which adds 1/S of a typical residual variance, while the
+// first two preparation passes read S conformations instead of n.  S grows
+// with n (n/8 above 1024) so that near-identical conformations of a large
+// ensemble keep residuals far below the noise of the rest.
+constexpr int kRefSampleMin = 128;
+__host__ __device__ inline int ref_sample_count(int n)
+{
+    const int s = n / 8 > kRefSampleMin ? n / 8 : kRefSampleMin;
+    return n < s ? n : s;
+}
+__host__ __device__ inline int ref_sample_index(int s, int n, int S)
+{
+    return (int)((int64_t)s * n / S);
+}
+
+// Reference of one ensemble (prep_device.hpp): for the S = ref_sample_count(n)
+// samples, centroids (cent, S x 3) and rotations onto conformation 0 (rot,
+// S x 9) as by-products, the slices' partial sums (partial, ceil(S/8) x 3L),
+// then r (ref: 3L values) and R0 (6).  cent and rot hold n entries; the fit
+// onto r overwrites them.
 void launch_reference(const double* xyz, int n, int L, double* rot, double* cent, double* partial,
                       double* ref, hipStream_t s);
 // Fit onto ref and pack: rotations (rot), centroids (cent), operand panels of
@@ -121,12 +143,13 @@ struct BatchEns {
     uint32_t* bits;         // flag bitmap of the n(n-1)/2 outputs
     float* out;             // packed triangle, f32
     int n, L, n_pad, n_kblk;
+    int n_ref;              // conformations sampled for the reference mean
     int n_slices, tile;     // reference-mean slices; tile side 32 or 64
     double acc_coef, rep_g_max;  // error-model thresholds at this L
 };
 
 // A batch: ensemble table, the prefix sums that map a batch-wide index
-// (conformation, padded conformation, mean slice, reference coordinate,
+// (reference sample, padded conformation, mean slice, reference coordinate,
 // bitmap word) to its ensemble, and the Gram work lists.
 //
 // Work items are dealt to 8 per-XCD lists (whole ensembles per list, costs
@@ -139,12 +162,12 @@ struct BatchArgs {
     const BatchEns* ens;
     int n_ens;
     int mode;
-    const int* conf_prefix;
+    const int* samp_prefix;  // reference-mean samples
     const int* pad_prefix;
     const int* slice_prefix;
     const int* coord_prefix;
     const int64_t* word_prefix;
-    int total_conf, total_pad, total_slices, total_coords;
+    int total_samp, total_pad, total_slices, total_coords;
     int64_t total_words;
     const uint32_t* items;
     const int* list_start;  // 9 entries: list x is items[list_start[x] .. list_start[x+1])

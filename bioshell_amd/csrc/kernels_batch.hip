@@ -1,9 +1,9 @@
 // kernels_batch.hip -- many ensembles of uneven size (n conformations, L
 // atoms each) in a fixed sequence of launches, independent of how many
 // ensembles there are: every preparation step is one launch over all the
-// batch's conformations (or mean slices, coordinates), the Gram step is one
-// persistent launch over a work list of (ensemble, tile) items, and the FP64
-// re-evaluation is one launch over all flag bitmaps.  The host captures the
+// batch's conformations (or reference samples, mean slices, coordinates),
+// the Gram step is one persistent launch over a work list of (ensemble, tile)
+// items, and the FP64 re-evaluation is one launch over all flag bitmaps.  The host captures the
 // sequence in a HIP graph.
 //
 // Batch-wide index -> (ensemble, local index) goes through the prefix sums of
@@ -36,36 +36,35 @@ __device__ inline int find_ens(const T* __restrict__ prefix, int n_ens, T g)
 __global__ __launch_bounds__(256) void batch_fit0_kernel(BatchArgs b)
 {
     const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= b.total_conf) return;
-    const int e = find_ens(b.conf_prefix, b.n_ens, g);
+    if (g >= b.total_samp) return;
+    const int e = find_ens(b.samp_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    const int i = g - b.conf_prefix[e];
-    fit0_one(E.xyz + (int64_t)i * E.L * 3, E.xyz, E.L, E.rot + (int64_t)i * 9,
-             E.cent + (int64_t)i * 3, threadIdx.x & 63);
+    const int t = g - b.samp_prefix[e];
+    const int i = ref_sample_index(t, E.n, E.n_ref);
+    fit0_one(E.xyz + (int64_t)i * E.L * 3, E.xyz, E.L, E.rot + (int64_t)t * 9,
+             E.cent + (int64_t)t * 3, threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(256) void batch_mean_partial_kernel(BatchArgs b)
 {
     __shared__ double sR[kMeanSliceConfs][9];
     __shared__ double sC[kMeanSliceConfs][3];
+    __shared__ int64_t sX[kMeanSliceConfs];
     const int g = blockIdx.x;
     const int e = find_ens(b.slice_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    mean_partial_block(E.xyz, E.cent, E.rot, E.n, E.L, g - b.slice_prefix[e], E.partial, sR, sC);
+    mean_partial_block(E.xyz, E.cent, E.rot, E.n, E.n_ref, E.L, g - b.slice_prefix[e], E.partial,
+                       sR, sC, sX);
 }
 
-// One thread per reference coordinate (batches hold few slices per ensemble;
-// the slices are added in order, as in the single-ensemble wave version).
+// One thread per reference coordinate, as in the single-ensemble kernel.
 __global__ __launch_bounds__(256) void batch_mean_final_kernel(BatchArgs b)
 {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= b.total_coords) return;
     const int e = find_ens(b.coord_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    const int v = g - b.coord_prefix[e];
-    double acc = 0.0;
-    for (int s = 0; s < E.n_slices; ++s) acc += E.partial[(int64_t)s * 3 * E.L + v];
-    E.ref[v] = acc / E.n;
+    mean_final_coord(E.partial, E.n_slices, E.n_ref, E.L, g - b.coord_prefix[e], E.ref);
 }
 
 __global__ __launch_bounds__(256) void batch_ref_gram_kernel(BatchArgs b)
@@ -254,7 +253,7 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(BatchArgs b)
 
 void launch_batch_prep(const BatchArgs& b, hipStream_t s)
 {
-    const int cw = (b.total_conf + 3) / 4;
+    const int cw = (b.total_samp + 3) / 4;
     batch_fit0_kernel<<<cw, 256, 0, s>>>(b);
     batch_mean_partial_kernel<<<b.total_slices, 256, 0, s>>>(b);
     batch_mean_final_kernel<<<(b.total_coords + 255) / 256, 256, 0, s>>>(b);

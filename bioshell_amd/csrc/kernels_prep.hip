@@ -6,36 +6,40 @@
 namespace bs {
 namespace {
 
-// Reference: centroid and fit onto conformation 0 for every conformation
-// (wave each), slices of the rotated sum, the mean (wave per coordinate),
+// Reference: centroid and fit onto conformation 0 for each of the S samples
+// (wave each), slices of the rotated sum, the mean (thread per coordinate,
+// one wave per work-group to spread the few coordinates over more CUs),
 // then R0 (one block).
-__global__ __launch_bounds__(256) void fit0_kernel(const double* __restrict__ xyz, int n, int L,
-                                                   double* __restrict__ rot,
+__global__ __launch_bounds__(256) void fit0_kernel(const double* __restrict__ xyz, int n, int S,
+                                                   int L, double* __restrict__ rot,
                                                    double* __restrict__ cent)
 {
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    fit0_one(xyz + (int64_t)i * L * 3, xyz, L, rot + (int64_t)i * 9, cent + (int64_t)i * 3,
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= S) return;
+    const int i = ref_sample_index(t, n, S);
+    fit0_one(xyz + (int64_t)i * L * 3, xyz, L, rot + (int64_t)t * 9, cent + (int64_t)t * 3,
              threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(256) void mean_partial_kernel(const double* __restrict__ xyz,
                                                            const double* __restrict__ cent,
                                                            const double* __restrict__ rot, int n,
-                                                           int L, double* __restrict__ partial)
+                                                           int S, int L,
+                                                           double* __restrict__ partial)
 {
     __shared__ double sR[kMeanSliceConfs][9];
     __shared__ double sC[kMeanSliceConfs][3];
-    mean_partial_block(xyz, cent, rot, n, L, blockIdx.x, partial, sR, sC);
+    __shared__ int64_t sX[kMeanSliceConfs];
+    mean_partial_block(xyz, cent, rot, n, S, L, blockIdx.x, partial, sR, sC, sX);
 }
 
-__global__ __launch_bounds__(256) void mean_final_kernel(const double* __restrict__ partial,
-                                                         int n_slices, int n, int L,
+__global__ __launch_bounds__(64) void mean_final_kernel(const double* __restrict__ partial,
+                                                         int n_slices, int S, int L,
                                                          double* __restrict__ ref)
 {
-    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int v = blockIdx.x * 64 + threadIdx.x;
     if (v >= 3 * L) return;
-    mean_final_wave(partial, n_slices, n, L, v, ref, threadIdx.x & 63);
+    mean_final_coord(partial, n_slices, S, L, v, ref);
 }
 
 __global__ __launch_bounds__(256) void ref_gram_kernel(double* __restrict__ ref, int L)
@@ -111,10 +115,11 @@ __global__ __launch_bounds__(64) void superpose_kernel(const double* __restrict_
 void launch_reference(const double* xyz, int n, int L, double* rot, double* cent, double* partial,
                       double* ref, hipStream_t s)
 {
-    const int slices = (n + kMeanSliceConfs - 1) / kMeanSliceConfs;
-    fit0_kernel<<<(n + 3) / 4, 256, 0, s>>>(xyz, n, L, rot, cent);
-    mean_partial_kernel<<<slices, 256, 0, s>>>(xyz, cent, rot, n, L, partial);
-    mean_final_kernel<<<(3 * L + 3) / 4, 256, 0, s>>>(partial, slices, n, L, ref);
+    const int S = ref_sample_count(n);
+    const int slices = (S + kMeanSliceConfs - 1) / kMeanSliceConfs;
+    fit0_kernel<<<(S + 3) / 4, 256, 0, s>>>(xyz, n, S, L, rot, cent);
+    mean_partial_kernel<<<slices, 256, 0, s>>>(xyz, cent, rot, n, S, L, partial);
+    mean_final_kernel<<<(3 * L + 63) / 64, 64, 0, s>>>(partial, slices, S, L, ref);
     ref_gram_kernel<<<1, 256, 0, s>>>(ref, L);
 }
 

@@ -2,10 +2,11 @@
 // as device functions shared by the single-ensemble kernels
 // (kernels_prep.hip) and the batched ones (kernels_batch.hip).
 //
-//   fit0     c_i = (sum x_i)/n (substructures.rs:27-42) and the rotation U_i
+//   fit0     for the S reference samples (kernels.hpp: ref_sample_count):
+//            c_i = (sum x_i)/n (substructures.rs:27-42) and the rotation U_i
 //            that superposes x_i - c_i onto x_0 - c_0
-//   mean     r = mean_i U_i (x_i - c_i): one Procrustes-mean step, r centred;
-//            then R0 = sum_k r_k r_k^T
+//   mean     r = mean over the samples of U_i (x_i - c_i): one
+//            Procrustes-mean step, r centred; then R0 = sum_k r_k r_k^T
 //   fit+pack U_i onto r from M = sum_k x_k r_k^T (with r centred, raw
 //            coordinates give the centred covariance), then
 //            e_i = U_i (x_i - c_i) - r split into MFMA operand panels, and
@@ -17,9 +18,9 @@
 // the key matrix's characteristic quartic by Newton from above, the
 // eigenvector from the adjugate of K - lambda I; every lane of the wave
 // computes it from the wave-reduced covariance, so no extra pass is needed.
-// Three passes stream the ensemble (24 bytes per atom each) and the pack
-// writes 12; all of it is O(n L) and runs once per ensemble, none of it per
-// pair.
+// The first two passes read the S samples (24 bytes per atom each), the fit
+// and pack reads all n conformations and writes 12 bytes per atom; all of it
+// is O(n L) and runs once per ensemble, none of it per pair.
 #pragma once
 
 #include "kernels.hpp"
@@ -232,25 +233,28 @@ __device__ inline void fit0_one(const double* __restrict__ x, const double* __re
     }
 }
 
-// Reference mean, stage 1: slice s (conformations 32 s .. 32 s + 31) of one
-// ensemble, each of the 256 threads one coordinate of the rotated centred sum
-// at a time.  sR (32 x 9) and sC (32 x 3) are LDS arrays.
+// Reference mean, stage 1: slice s (samples 8 s .. 8 s + 7 of the S) of an
+// ensemble of n conformations, each of the 256 threads one coordinate of the
+// rotated centred sum at a time.  Sample t's rotation and centroid are
+// entry t of rot and cent.  sR (8 x 9), sC (8 x 3) and sX (8 sample
+// offsets) are LDS arrays.
 __device__ inline void mean_partial_block(const double* __restrict__ xyz,
                                           const double* __restrict__ cent,
-                                          const double* __restrict__ rot, int n, int L, int s,
-                                          double* __restrict__ partial, double (*sR)[9],
-                                          double (*sC)[3])
+                                          const double* __restrict__ rot, int n, int S, int L,
+                                          int s, double* __restrict__ partial, double (*sR)[9],
+                                          double (*sC)[3], int64_t* sX)
 {
     const int i0 = s * kMeanSliceConfs;
-    const int cnt = min(kMeanSliceConfs, n - i0);
+    const int cnt = min(kMeanSliceConfs, S - i0);
     for (int v = threadIdx.x; v < cnt * 9; v += 256) sR[v / 9][v % 9] = rot[(int64_t)i0 * 9 + v];
     for (int v = threadIdx.x; v < cnt * 3; v += 256) sC[v / 3][v % 3] = cent[(int64_t)i0 * 3 + v];
+    if (threadIdx.x < cnt) sX[threadIdx.x] = (int64_t)ref_sample_index(i0 + threadIdx.x, n, S) * L * 3;
     __syncthreads();
     for (int v = threadIdx.x; v < 3 * L; v += 256) {
         const int k = v / 3, c = v - 3 * (v / 3);
         double acc = 0.0;
         for (int t = 0; t < cnt; ++t) {
-            const double* x = xyz + ((int64_t)(i0 + t) * L + k) * 3;
+            const double* x = xyz + sX[t] + 3 * k;
             acc += sR[t][3 * c] * (x[0] - sC[t][0]) + sR[t][3 * c + 1] * (x[1] - sC[t][1]) +
                    sR[t][3 * c + 2] * (x[2] - sC[t][2]);
         }
@@ -258,15 +262,24 @@ __device__ inline void mean_partial_block(const double* __restrict__ xyz,
     }
 }
 
-// Stage 2: one wave adds coordinate v over the slices in a fixed order
-// (deterministic, no atomics).
-__device__ inline void mean_final_wave(const double* __restrict__ partial, int n_slices, int n,
-                                       int L, int v, double* __restrict__ ref, int lane)
+// Stage 2: coordinate v of r by one thread, in a fixed order (no atomics;
+// the single-ensemble and batched kernels share it, so both give the same
+// r): eight running sums over the slices s = q mod 8, so that eight loads
+// are in flight at a time, then added pairwise.
+__device__ inline void mean_final_coord(const double* __restrict__ partial, int n_slices, int S,
+                                        int L, int v, double* __restrict__ ref)
 {
-    double acc = 0.0;
-    for (int s = lane; s < n_slices; s += 64) acc += partial[(int64_t)s * 3 * L + v];
-    acc = wave_sum(acc);
-    if (lane == 0) ref[v] = acc / n;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double* p = partial + v;
+    const int64_t ld = 3 * (int64_t)L;
+    int s = 0;
+    for (; s + 8 <= n_slices; s += 8)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] += p[(s + q) * ld];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (s + q < n_slices) acc[q] += p[(s + q) * ld];
+    ref[v] = (((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]))) / S;
 }
 
 // Work-group ids are dealt to the 8 XCDs in turn.  A pack grid of 8 * per
