@@ -80,6 +80,18 @@ def test_batch_bit_identical_to_single(tmp_path, batch, tile):
     ctx.close()
 
 
+def test_batch_large_ensemble_sampled_reference(gpu_ctx):
+    """An ensemble above 1024 conformations takes its reference mean over n/8
+    samples, in the batch as in the single call: both give the same triangle,
+    within the gate of the oracle."""
+    xs = make_batch(700, [(1100, 24), (300, 40), (2, 9)])
+    got = B.crmsd_batch(xs, B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    for x, g in zip(xs, got):
+        single = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx).upper
+        assert np.array_equal(g, single), f"shape {x.shape}"
+        assert np.abs(g.astype(np.float64) - O.all_vs_all(x, nthreads=8)).max() <= TOL_GRAM
+
+
 def test_batch_graph_replay_and_rebuild(gpu_ctx, batch):
     """The captured graph is replayed for a repeated shape and rebuilt for a new
     one; every replay gives the eager-equivalent (first) result."""

@@ -130,6 +130,25 @@ def test_near_identical_pairs(gpu_ctx):
         assert err[small].max() <= TOL[method]
 
 
+def test_near_identical_large_ensemble_not_flagged(gpu_ctx):
+    """4,000 conformations: the shifted frame's reference is the mean of 500
+    evenly spaced samples.  The near-identical conformations' residuals are
+    that mean's noise (about 0.1 A); the error model keeps their pairs out of
+    FP64 re-evaluation, and every value still meets the gate."""
+    x = B.generate_ensemble(12, 4000, 100, sigma=(0.5, 4.0), near_fraction=0.05, near_sigma=0.01)
+    got = run(gpu_ctx, x, B.Method.BF16X3_GRAM)
+    flagged = gpu_ctx.last_stats()["flagged_pairs"]
+    want = O.all_vs_all(x, nthreads=8)
+    err = np.abs(got - want)
+    report("near-identical n=4000 BF16X3_GRAM", err)
+    print("flagged", flagged, flush=True)
+    assert err.max() <= TOL[B.Method.BF16X3_GRAM]
+    small = want < 0.1
+    assert small.sum() > 10000
+    assert err[small].max() <= TOL[B.Method.BF16X3_GRAM]
+    assert flagged == 0
+
+
 def test_duplicates_far_from_reference(gpu_ctx):
     """Exact and near duplicates of strongly perturbed conformations: the split
     products' error is largest here, so these pairs must be re-evaluated."""
