@@ -237,24 +237,15 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     rec->base.main_launches = 1;
     rec->base.tile_m = rec->base.tile_n = 0;
     if (E == 0) {
-        for (int k = 0; k < 4; ++k) {
-            rec->start(k, s);
-            rec->stop(k, s);
-        }
+        rec->mark({kBTotal0, kBTotal1, kBPrep0, kBPrep1, kBMain0, kBMain1, kBFallback0, kBFallback1}, s);
         return;
     }
     if (!P.gram) {
-        rec->start(kRecTotal, s);
-        rec->start(kRecPrep, s);
-        rec->stop(kRecPrep, s);
-        rec->start(kRecMain, s);
+        rec->mark({kBTotal0, kBPrep0, kBPrep1, kBMain0}, s);
         for (int e = 0; e < E; ++e)
             launch_plain_avsa(d_xyz[e], P.n[e], P.L[e], P.method == BS_FP64_PLAIN, d_out[e], false, s);
-        rec->stop(kRecMain, s);
         BS_HIP(hipGetLastError());
-        rec->start(kRecFallback, s);
-        rec->stop(kRecFallback, s);
-        rec->stop(kRecTotal, s);
+        rec->mark({kBMain1, kBFallback0, kBFallback1, kBTotal1}, s);
         rec->base.main_launches = (uint64_t)E;
         return;
     }
@@ -375,13 +366,10 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
         W.exec_key.swap(key);
         ++W.graph_builds;
     }
-    rec->start(kRecTotal, s);
-    for (int k = 1; k < 4; ++k) {  // stages run inside the graph: not timed separately
-        rec->start(k, s);
-        rec->stop(k, s);
-    }
+    // the stages run inside the graph: only the total is timed
+    rec->mark({kBTotal0, kBPrep0, kBPrep1, kBMain0, kBMain1, kBFallback0, kBFallback1}, s);
     BS_HIP(hipGraphLaunch(W.exec, s));
-    rec->stop(kRecTotal, s);
+    rec->mark({kBTotal1}, s);
     ++W.graph_launches;
     if (rec->h_count)
         BS_HIP(hipMemcpyAsync(rec->h_count, a.counters + kLists, 4, hipMemcpyDeviceToHost, s));

@@ -87,14 +87,30 @@ void RunRecord::destroy()
         if (e) (void)hipEventDestroy(e);
 }
 
+void RunRecord::reset()
+{
+    for (auto& x : slot) x = -1;
+    n_ev = 0;
+}
+
+void RunRecord::mark(std::initializer_list<int> bounds, hipStream_t s)
+{
+    BS_REQUIRE(n_ev < 8, "too many events in one run record");
+    BS_HIP(hipEventRecord(ev[n_ev], s));
+    for (int b : bounds) slot[b] = (int8_t)n_ev;
+    ++n_ev;
+}
+
 bs_run_stats RunRecord::read() const
 {
     bs_run_stats st = base;
-    if (!used) return st;
-    BS_HIP(hipEventSynchronize(ev[2 * kRecTotal + 1]));
+    if (!used || n_ev == 0) return st;
+    BS_HIP(hipEventSynchronize(ev[n_ev - 1]));
     auto span = [&](int k) {
+        const int a = slot[2 * k], b = slot[2 * k + 1];
+        if (a < 0 || b < 0 || a == b) return 0.0;
         float v = 0.0f;
-        BS_HIP(hipEventElapsedTime(&v, ev[2 * k], ev[2 * k + 1]));
+        BS_HIP(hipEventElapsedTime(&v, ev[a], ev[b]));
         return (double)v;
     };
     st.total_ms = span(kRecTotal);
@@ -170,23 +186,27 @@ void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScra
     g.max_wg = sc.max_wg > 0 ? sc.max_wg : 512;
     g.skew_sleeps = sc.skew_sleeps;
 
-    if (rec) rec->start(kRecMain, s);
+    if (rec) rec->mark({kBPrep1, kBMain0}, s);  // the bitmap clear counts as preparation
     if (sc.variant == 1)
         launch_gram_pipe(g, A.mode, s);
     else
         launch_gram(g, A.mode, s);
-    if (rec) rec->stop(kRecMain, s);
+    if (rec) rec->mark({kBMain1, kBFallback0}, s);
     BS_HIP(hipGetLastError());
-    if (rec) rec->start(kRecFallback, s);
     launch_fallback(A.xyz, A.cent, B.xyz, B.cent, A.L, flags,
                     (int64_t)n_out, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, list,
                     kFlagListCap, s);
-    if (rec) rec->stop(kRecFallback, s);
     BS_HIP(hipGetLastError());
     if (rec && rec->h_count)
         BS_HIP(hipMemcpyAsync(rec->h_count, count, 4, hipMemcpyDeviceToHost, s));
     if (run.h_count) BS_HIP(hipMemcpyAsync(run.h_count, count, 4, hipMemcpyDeviceToHost, s));
-    if (rec) rec->gram = true;
+    if (rec) {
+        if (run.ends_total)
+            rec->mark({kBFallback1, kBTotal1}, s);
+        else
+            rec->mark({kBFallback1}, s);
+        rec->gram = true;
+    }
 }
 
 void GramScratch::reset()
@@ -258,36 +278,27 @@ void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m
     hipStream_t s = ctx->stream;
     RunRecord* rec = ctx->begin_record();
     rec->base.pairs = n >= 2 ? (uint64_t)n * (n - 1) / 2 : 0;
-    rec->start(kRecTotal, s);
     if (n >= 2 && !is_gram(m)) {
-        rec->start(kRecPrep, s);
-        rec->stop(kRecPrep, s);
-        rec->start(kRecMain, s);
+        rec->mark({kBTotal0, kBPrep0, kBPrep1, kBMain0}, s);
         launch_plain_avsa(d_xyz, n, L, m == BS_FP64_PLAIN, d_out, out_f64, s);
-        rec->stop(kRecMain, s);
         BS_HIP(hipGetLastError());
-        rec->start(kRecFallback, s);
-        rec->stop(kRecFallback, s);
+        rec->mark({kBMain1, kBFallback0, kBFallback1, kBTotal1}, s);
         rec->base.main_launches = 1;
     } else if (n >= 2) {
-        rec->start(kRecPrep, s);
+        rec->mark({kBTotal0, kBPrep0}, s);
         prepare(ctx->prep, d_xyz, n, L, mode_of(m), nullptr, s);
-        rec->stop(kRecPrep, s);
         GramRun run;
         run.triangle = true;
         run.d_out = d_out;
         run.out_f64 = out_f64;
+        run.ends_total = true;
         run_gram(view_of(ctx->prep), view_of(ctx->prep), run, ctx->gsc, s, rec);
         rec->base.main_launches = 1;
         rec->base.tile_m = kTileRows;
         rec->base.tile_n = kTileRows;
     } else {
-        for (int k = 1; k < 4; ++k) {
-            rec->start(k, s);
-            rec->stop(k, s);
-        }
+        rec->mark({kBTotal0, kBTotal1, kBPrep0, kBPrep1, kBMain0, kBMain1, kBFallback0, kBFallback1}, s);
     }
-    rec->stop(kRecTotal, s);
 }
 
 }  // namespace bs
@@ -304,7 +315,8 @@ bs::RunRecord* bs_context::begin_record()
     bs::RunRecord* r = &ring[ring_next];
     ring_next = (ring_next + 1) % ring.size();
     ring_used = std::min(ring_used + 1, ring.size());
-    if (r->used) BS_HIP(hipEventSynchronize(r->ev[2 * bs::kRecTotal + 1]));  // never reuse in flight
+    if (r->used && r->n_ev > 0) BS_HIP(hipEventSynchronize(r->ev[r->n_ev - 1]));  // never reuse in flight
+    r->reset();
     r->base = bs_run_stats{};
     r->gram = false;
     r->used = true;

@@ -1,6 +1,8 @@
 // engine.hpp -- host orchestration shared by the ABI entry points.
 #pragma once
 
+#include <initializer_list>
+
 #include <vector>
 
 #include "batch.hpp"
@@ -73,6 +75,7 @@ struct GramRun {
     void* d_out = nullptr;
     bool out_f64 = false;
     int64_t out_ld = 0;
+    bool ends_total = false;      // the record's total ends with this run
 };
 
 // Scratch a Gram run needs besides the prepared sets.
@@ -109,19 +112,30 @@ struct GramScratch {
 };
 
 // Events and host-visible results of one call, read lazily.
+// Stage boundaries of a call's record: 2k starts stage k, 2k + 1 ends it.
+enum { kRecTotal = 0, kRecPrep = 1, kRecMain = 2, kRecFallback = 3 };
+enum {
+    kBTotal0 = 0, kBTotal1 = 1, kBPrep0 = 2, kBPrep1 = 3,
+    kBMain0 = 4, kBMain1 = 5, kBFallback0 = 6, kBFallback1 = 7
+};
+
+// Timing of one call.  Each mark() records one event for the boundaries it
+// names, so boundaries with no work between them share an event: an event
+// record costs the stream a few microseconds, as much as a small kernel.
 struct RunRecord {
-    hipEvent_t ev[8] = {};       // total, prep, main, fallback: start/stop pairs
+    hipEvent_t ev[8] = {};        // in the order recorded
+    int8_t slot[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // boundary -> index into ev
+    int n_ev = 0;
     uint32_t* h_count = nullptr;  // flagged-pair count (pinned), copied after the call
     bs_run_stats base{};          // pairs, launches, tile sizes
     bool gram = false;
     bool used = false;
     void create();
     void destroy();
-    void start(int k, hipStream_t s) { BS_HIP(hipEventRecord(ev[2 * k], s)); }
-    void stop(int k, hipStream_t s) { BS_HIP(hipEventRecord(ev[2 * k + 1], s)); }
+    void reset();
+    void mark(std::initializer_list<int> bounds, hipStream_t s);
     bs_run_stats read() const;  // synchronises on the last event
 };
-enum { kRecTotal = 0, kRecPrep = 1, kRecMain = 2, kRecFallback = 3 };
 
 // Gram contraction + QCP epilogue + FP64 re-evaluation of flagged pairs over
 // rows of A x cols of B (or the upper triangle of A when triangle).  Asynchronous:

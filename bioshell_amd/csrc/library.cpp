@@ -79,10 +79,8 @@ void enqueue_query(bs_library* lib, QueryWorkspace& ws, const double* q_host, in
     float* scores =
         static_cast<float*>(ws.scores.ensure((size_t)n_q * n_lib * 4, "query scores"));
     BS_HIP(hipMemcpyAsync(dq, q_host, (size_t)n_q * L * 3 * 8, hipMemcpyHostToDevice, s));
-    if (rec) rec->start(kRecTotal, s);
-    if (rec) rec->start(kRecPrep, s);
+    if (rec) rec->mark({kBTotal0, kBPrep0}, s);
     prepare(ws.prep, dq, n_q, L, lib->P.mode, lib->P.ref.as<double>(), s);
-    if (rec) rec->stop(kRecPrep, s);
     GramRun run;
     run.triangle = false;
     run.d_out = scores;
@@ -97,7 +95,7 @@ void enqueue_query(bs_library* lib, QueryWorkspace& ws, const double* q_host, in
         launch_topk(scores, n_q, n_lib, k, idx, val, idx + (size_t)n_q * k * 2, s);
         BS_HIP(hipGetLastError());
     }
-    if (rec) rec->stop(kRecTotal, s);
+    if (rec) rec->mark({kBTotal1}, s);
     if (rec) {
         rec->base.pairs = (uint64_t)n_q * n_lib;
         rec->base.main_launches = 1;

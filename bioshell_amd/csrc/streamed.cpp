@@ -108,11 +108,9 @@ bs_status bs_crmsd_all_vs_all_streamed(bs_context* ctx, const double* xyz, size_
         rec->base.pairs = (uint64_t)n * (n - 1) / 2;
         rec->base.main_launches = (uint64_t)K;
         rec->base.tile_m = rec->base.tile_n = T;
-        rec->start(kRecTotal, s);
-        rec->start(kRecPrep, s);
+        rec->mark({kBTotal0, kBPrep0}, s);
         prepare(ctx->prep, d, n, L, mode_of(method), nullptr, s);
-        rec->stop(kRecPrep, s);
-        rec->start(kRecMain, s);
+        rec->mark({kBPrep1, kBMain0}, s);
         auto drain = [&](int k) {  // block k's copy has been issued: wait, scatter
             BS_HIP(hipEventSynchronize(ctx->ev_copy[k % 2]));
             scatter_block(n, T, blocks[k].first, blocks[k].second,
@@ -138,10 +136,7 @@ bs_status bs_crmsd_all_vs_all_streamed(bs_context* ctx, const double* xyz, size_
             BS_HIP(hipMemcpyAsync(ctx->h_blk[slot], dev[slot], bytes, hipMemcpyDeviceToHost, cs));
             BS_HIP(hipEventRecord(ctx->ev_copy[slot], cs));
         }
-        rec->stop(kRecMain, s);
-        rec->start(kRecFallback, s);
-        rec->stop(kRecFallback, s);
-        rec->stop(kRecTotal, s);
+        rec->mark({kBMain1, kBFallback0, kBFallback1, kBTotal1}, s);
         for (int k = std::max(0, K - 2); k < K; ++k) drain(k);
         BS_HIP(hipStreamSynchronize(s));
         uint64_t flagged = 0;

@@ -68,12 +68,13 @@ typedef struct bs_library bs_library;
 typedef struct bs_server bs_server;
 
 /* Timings and counters of the last all-vs-all / batch / query call on a
- * context.  Times are device times from HIP events on the context's stream. */
+ * context.  Times are device times from HIP events on the context's stream
+ * (adjacent stage boundaries share one event; a batch times only its total). */
 typedef struct bs_run_stats {
     double total_ms;        /* first prep kernel start .. last kernel end      */
-    double prep_ms;         /* centring, reference fit, operand packing        */
+    double prep_ms;         /* centring, reference fit, packing, flag clear    */
     double main_ms;         /* the pair kernel(s): plain or Gram+QCP            */
-    double fallback_ms;     /* FP64 re-evaluation of flagged pairs             */
+    double fallback_ms;     /* FP64 re-evaluation of flagged pairs, count copy */
     uint64_t pairs;         /* pairs evaluated                                 */
     uint64_t flagged_pairs; /* pairs re-evaluated in FP64                       */
     uint64_t main_launches; /* launches of the main pair kernel                 */
