@@ -34,18 +34,18 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
     P.xyz = d_xyz;
     double* cent = static_cast<double*>(P.cent.ensure((size_t)n * 3 * 8, "centroids"));
     double* rot = static_cast<double*>(P.rot.ensure((size_t)n * 9 * 8, "rotations"));
-    double* ref = static_cast<double*>(P.ref.ensure((size_t)(3 * L + 6) * 8, "reference"));
     void* panels = P.panels.ensure((size_t)P.n_pad / kConfBlock * P.n_kblk * kPanelBlockU4 * 16,
                                    "operand panels");
     double* consts = static_cast<double*>(P.consts.ensure((size_t)kNumConsts * P.n_pad * 8, "constants"));
 
-    if (fixed_ref) {
-        BS_HIP(hipMemcpyAsync(ref, fixed_ref, (size_t)(3 * L + 6) * 8, hipMemcpyDeviceToDevice, s));
-    } else {
+    const double* ref = fixed_ref;  // a fixed reference is read in place
+    if (!fixed_ref) {
+        double* own = static_cast<double*>(P.ref.ensure((size_t)(3 * L + 6) * 8, "reference"));
         const size_t slices = (size_t)(ref_sample_count(n) + kMeanSliceConfs - 1) / kMeanSliceConfs;
         double* partial = static_cast<double*>(
             P.partial.ensure(slices * 3 * L * 8, "reference partial sums"));
-        launch_reference(d_xyz, n, L, rot, cent, partial, ref, s);
+        launch_reference(d_xyz, n, L, rot, cent, partial, own, s);
+        ref = own;
     }
     launch_fit_pack(d_xyz, ref, n, P.n_pad, L, P.n_kblk, mode, rot, cent, panels, consts, P.n_pad, s);
     BS_HIP(hipGetLastError());

@@ -57,8 +57,8 @@ size_t prepared_bytes(int n, int L);
 
 // Reference, fit, pack (prep_device.hpp).  If `fixed_ref` (device, 3L + 6
 // doubles, the layout of Prepared::ref) is given, conformations are fitted
-// onto it and it is copied as this set's reference; otherwise the reference is
-// one Procrustes-mean step from conformation 0.
+// onto it, read in place (Prepared::ref stays unused); otherwise the reference
+// is one Procrustes-mean step from conformation 0 over the reference samples.
 // d_xyz is borrowed (Prepared::xyz).
 void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
              const double* fixed_ref, hipStream_t s);

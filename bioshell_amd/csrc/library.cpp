@@ -56,7 +56,6 @@ void QueryWorkspace::reserve(int n_q, int n_lib, int L, int k)
     const int n_pad = round_up(n_q, 64), n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
     prep.cent.ensure((size_t)n_q * 3 * 8, "centroids");
     prep.rot.ensure((size_t)n_q * 9 * 8, "rotations");
-    prep.ref.ensure((size_t)(3 * L + 6) * 8, "reference");
     prep.panels.ensure((size_t)n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16, "operand panels");
     prep.consts.ensure((size_t)kNumConsts * n_pad * 8, "constants");
     xyz.ensure((size_t)n_q * L * 3 * 8, "query coordinates");
