@@ -58,7 +58,6 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         ctx->device = device;
         ctx->n_cu = prop.multiProcessorCount;
         ctx->gsc.max_wg = 2 * ctx->n_cu;
-        if (const char* e = std::getenv("BIOSHELL_GRAM_SKEW")) ctx->gsc.skew_sleeps = std::atoi(e);
         if (const char* e = std::getenv("BIOSHELL_GRAM_VARIANT")) ctx->gsc.variant = std::atoi(e);
         ctx->budget.set_cap(hbm_budget_bytes);
         BS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));

@@ -184,7 +184,6 @@ void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScra
     g.tau = th.tau;
     g.rep_g_max = th.rep_g_max;
     g.max_wg = sc.max_wg > 0 ? sc.max_wg : 512;
-    g.skew_sleeps = sc.skew_sleeps;
 
     if (rec) rec->mark({kBPrep1, kBMain0}, s);  // the bitmap clear counts as preparation
     if (sc.variant == 1)

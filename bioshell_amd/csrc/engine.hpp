@@ -92,7 +92,6 @@ struct GramScratch {
     TileOrder orders[kOrders];
     int next_order = 0;
     int max_wg = 0;               // persistent Gram work-groups (2 per CU)
-    int skew_sleeps = 0;          // phase offset of the second half of the grid (simple variant)
     int variant = 0;              // 0: one tile per work-group, 1: software-pipelined, persistent
     uint32_t* h_order = nullptr;  // pinned staging of a tile order being uploaded
     size_t h_order_cap = 0;

@@ -93,7 +93,6 @@ struct GramArgs {
     double rep_g_max;       // g_i + g_j above which the split's rounding of e may show
     // launch shape
     int max_wg;             // resident work-groups (2 per CU)
-    int skew_sleeps;        // s_sleep(127) count delaying ids [max_wg/2, max_wg)
 };
 
 // Builds the logical tile order (XCD-aware super-blocks) on the host.

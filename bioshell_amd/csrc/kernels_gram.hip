@@ -39,8 +39,6 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
     const int per = (a.n_tiles + kXcd - 1) / kXcd;
     const int t = (wg % kXcd) * per + (wg / kXcd);
     if (t >= a.n_tiles) return;
-    if (a.skew_sleeps > 0 && wg >= a.max_wg / 2 && wg < a.max_wg)
-        for (int k = 0; k < a.skew_sleeps; ++k) __builtin_amdgcn_s_sleep(127);
     const uint32_t code = a.tiles[t];
     gram_tile64<MODE>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds, (int)threadIdx.x);
 }
