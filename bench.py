@@ -652,6 +652,12 @@ def main() -> None:
             "sample": f"rows 0..{R - 1} of the same 10,000 x 300 ensemble ({cb['pairs']} pairs), "
                       f"FP64 QCP oracle, {cb['seconds']:.1f} s",
         }
+        one = cpu_baseline(x, args.cpu_seconds / 2, 1)  # the same oracle on one thread
+        cpu["one_thread"] = {
+            "value": round(one["value"], 1),
+            "cores": 1,
+            "sample": f"rows 0..{one['R'] - 1} ({one['pairs']} pairs), {one['seconds']:.1f} s",
+        }
 
     nxt = None if args.no_next else next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x)
     batch = None if args.no_batch else batch_leg(args, B, torch, log)
