@@ -84,7 +84,10 @@ def test_batch_large_ensemble_sampled_reference(gpu_ctx):
     """An ensemble above 1024 conformations takes its reference mean over n/8
     samples, in the batch as in the single call: both give the same triangle,
     within the gate of the oracle."""
-    xs = make_batch(700, [(1100, 24), (300, 40), (2, 9)])
+    # sample counts around their switch points: S = n up to 128, then 128,
+    # then n / 8 above 1024 (129 at 1032 and 1033)
+    xs = make_batch(700, [(1100, 24), (300, 40), (2, 9), (128, 20), (129, 20), (1024, 10),
+                          (1025, 10), (1033, 10)])
     got = B.crmsd_batch(xs, B.Method.BF16X3_GRAM, ctx=gpu_ctx)
     for x, g in zip(xs, got):
         single = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx).upper
