@@ -183,6 +183,17 @@ __device__ inline bool better(const Cand& a, const Cand& b)
     return a.i < b.i;
 }
 
+// The order of better() as one unsigned key: the order-preserving bits of v
+// (-0 taken as +0, which compares equal to it), then j, then i, each index in
+// 16 bits (n <= 65536).  No candidate (j = 0x7FFFFFFF) is the largest key.
+// NaN never becomes a candidate: a scan takes only values below f32::MAX.
+__device__ inline uint64_t cand_key(const Cand& c)
+{
+    if (c.j == 0x7FFFFFFF) return ~0ull;
+    const float v = c.v == 0.0f ? 0.0f : c.v;
+    return ((uint64_t)fkey(__float_as_uint(v)) << 32) | ((uint64_t)(unsigned)c.j << 16) | (unsigned)c.i;
+}
+
 __global__ void dense_from_upper_kernel(const float* __restrict__ up, int n, int ld, float* __restrict__ D,
                                         unsigned* __restrict__ sizes)
 {
@@ -392,58 +403,33 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict_
                                                               unsigned* __restrict__ mj,
                                                               float* __restrict__ md)
 {
-    __shared__ Cand red[1024];
-    __shared__ int s_i, s_j;
-    __shared__ unsigned s_si, s_sj;
-    __shared__ float s_dij;
+    __shared__ uint64_t wkey[16];
     const int s = st->s;  // read by every thread before thread 0 advances it below
     if (s >= n - 1) return;
-    Cand b{3.402823466e+38f, 0x7FFFFFFF, 0x7FFFFFFF};
-    for (int k = threadIdx.x; k < n_part; k += 1024) {
-        const Cand c = part[k];
-        if (better(c, b)) b = c;
-    }
-    red[threadIdx.x] = b;
+    // the minimum key over the candidates: one wave reduction each, then the
+    // 16 wave minima from LDS in every thread
+    uint64_t key = ~0ull;
+    for (int k = threadIdx.x; k < n_part; k += 1024) key = min(key, cand_key(part[k]));
+    key = wave_min_u64(key);
+    if ((threadIdx.x & 63) == 0) wkey[threadIdx.x >> 6] = key;
     __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-        if (threadIdx.x < h && better(red[threadIdx.x + h], red[threadIdx.x])) red[threadIdx.x] = red[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        Cand c = red[0];
-        if (c.j == 0x7FFFFFFF) {  // every value >= f32::MAX: the reference keeps (0, 0)
-            c.i = 0;
-            c.j = 0;
-        }
-        const unsigned si = sizes[c.i], sj = sizes[c.j];
-        const float dij = D[(int64_t)c.i * ld + c.j];
-        s_i = c.i;
-        s_j = c.j;
-        s_si = si;
-        s_sj = sj;
-        s_dij = dij;
-        st->i = c.i;
-        st->j = c.j;
-        st->si = si;
-        st->sj = sj;
-        st->dij = dij;
-        sizes[c.i] = sj + si;
-        mi[s] = (unsigned)c.i;
-        mj[s] = (unsigned)c.j;
-        md[s] = dij;
-        st->s = s + 1;
-    }
-    __syncthreads();  // every read of the old rows below follows thread 0's read of D[i][j]
-    const int i = s_i, j = s_j;
+    key = wkey[0];
+#pragma unroll
+    for (int w = 1; w < 16; ++w) key = min(key, wkey[w]);
+    // every thread loads, in one round, the old values it needs: the pair's
+    // sizes and distance, both rows' entries at the last active index and its
+    // first groups of rows i, j and last.  No candidate (every value >=
+    // f32::MAX): the reference keeps (0, 0).
+    const int i = key == ~0ull ? 0 : (int)(key & 0xFFFFu);
+    const int j = key == ~0ull ? 0 : (int)((key >> 16) & 0xFFFFu);
     const int order = n - s - 1;  // active elements after this merge; index `order` is the last
     const bool moved = j < order;
-    const unsigned si = s_si, sj = s_sj;
-    const float dij = s_dij;
     float* Ri = D + (int64_t)i * ld;
     float* Rj = D + (int64_t)j * ld;
     const float* Rl = D + (int64_t)order * ld;
-    // res[last]: the merged cluster to the element that moves into j
-    const float rl = moved ? merge_rule(rule, si, sj, sj, dij, Ri[order], Rj[order]) : 0.0f;
+    const unsigned si = sizes[i], sj = sizes[j];
+    const float dij = Ri[j];
+    const float ri_last = Ri[order], rj_last = Rj[order];
     // four columns per thread and group, two groups per trip with every load
     // issued before any store (rows are 16-byte aligned; columns at or past
     // `order` keep their values)
@@ -451,8 +437,8 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict_
     float4* Rj4 = reinterpret_cast<float4*>(Rj);
     const float4* Rl4 = reinterpret_cast<const float4*>(Rl);
     const int g_end = (order + 3) >> 2;
-    for (int g0 = threadIdx.x; g0 < g_end; g0 += 2048) {
-        float4 a[2], b4[2], l4[2];
+    float4 a[2], b4[2], l4[2];
+    auto load_trip = [&](int g0) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int g = g0 + 1024 * u;
@@ -462,6 +448,26 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict_
                 l4[u] = Rl4[g];
             }
         }
+    };
+    load_trip(threadIdx.x);
+    // every read of a value that this kernel overwrites (D[i][j] is in row i,
+    // sizes[i]) is done before any store
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->i = i;
+        st->j = j;
+        st->si = si;
+        st->sj = sj;
+        st->dij = dij;
+        sizes[i] = sj + si;
+        mi[s] = (unsigned)i;
+        mj[s] = (unsigned)j;
+        md[s] = dij;
+        st->s = s + 1;
+    }
+    // res[last]: the merged cluster to the element that moves into j
+    const float rl = moved ? merge_rule(rule, si, sj, sj, dij, ri_last, rj_last) : 0.0f;
+    auto store_trip = [&](int g0) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int g = g0 + 1024 * u;
@@ -472,23 +478,28 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict_
             float ni[4], nj[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int c = 4 * g + k;
-                if (c >= order) {
+                const int col = 4 * g + k;
+                if (col >= order) {
                     ni[k] = oi[k];
                     nj[k] = oj[k];
                     continue;
                 }
-                if (c == i)
+                if (col == i)
                     ni[k] = 0.0f;
-                else if (moved && c == j)
+                else if (moved && col == j)
                     ni[k] = rl;
                 else
                     ni[k] = merge_rule(rule, si, sj, sj, dij, oi[k], oj[k]);
-                nj[k] = c == j ? 0.0f : (c == i ? rl : ol[k]);
+                nj[k] = col == j ? 0.0f : (col == i ? rl : ol[k]);
             }
             Ri4[g] = make_float4(ni[0], ni[1], ni[2], ni[3]);
             if (moved) Rj4[g] = make_float4(nj[0], nj[1], nj[2], nj[3]);  // with the (0, 0) pair the move wins
         }
+    };
+    store_trip(threadIdx.x);
+    for (int g0 = threadIdx.x + 2048; g0 < g_end; g0 += 2048) {
+        load_trip(g0);
+        store_trip(g0);
     }
 }
 

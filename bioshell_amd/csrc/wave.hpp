@@ -65,6 +65,30 @@ __device__ inline unsigned wave_min_u32(unsigned v)
     return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// Minimum of a 64-bit unsigned value over the 64 lanes (all active), in
+// every lane: the same pattern on both halves, compared as one value.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_min_u64(uint64_t v)
+{
+    const unsigned lo = dpp_u32<CTRL, ROW_MASK>((unsigned)v, 0xFFFFFFFFu);
+    const unsigned hi = dpp_u32<CTRL, ROW_MASK>((unsigned)(v >> 32), 0xFFFFFFFFu);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    return o < v ? o : v;
+}
+
+__device__ inline uint64_t wave_min_u64(uint64_t v)
+{
+    v = dpp_min_u64<0x111, 0xF>(v);
+    v = dpp_min_u64<0x112, 0xF>(v);
+    v = dpp_min_u64<0x114, 0xF>(v);
+    v = dpp_min_u64<0x118, 0xF>(v);
+    v = dpp_min_u64<0x142, 0xA>(v);
+    v = dpp_min_u64<0x143, 0xC>(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ inline unsigned wave_max_u32(unsigned v)
 {
     v = max(v, dpp_u32<0x111, 0xF>(v, 0u));
