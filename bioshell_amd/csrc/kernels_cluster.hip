@@ -169,29 +169,17 @@ __global__ __launch_bounds__(256) void members_row_max_kernel(const float* __res
 }
 
 // ------------------------------------------------------------------ clustering
-struct Cand {
-    float v;
-    int j, i;
-};
-
-// (v, j, i) lexicographic: the first strict minimum of the reference's scan
-// (j ascending, then i ascending; clustering_matrix.rs:27-42).
-__device__ inline bool better(const Cand& a, const Cand& b)
+// A candidate pair (v = D[j][i], i < j) as one unsigned key whose order is (v, j, i)
+// lexicographic, the first strict minimum of the reference's scan (j
+// ascending, then i ascending; clustering_matrix.rs:27-42): the
+// order-preserving bits of v (-0 taken as +0, which compares equal to it),
+// then j, then i, each index in 16 bits (n <= 65536).  No candidate is the
+// largest key, ~0.  NaN never becomes a candidate: a scan takes only values
+// below f32::MAX.
+__device__ inline uint64_t cand_key(float v, int j, int i)
 {
-    if (a.v != b.v) return a.v < b.v;
-    if (a.j != b.j) return a.j < b.j;
-    return a.i < b.i;
-}
-
-// The order of better() as one unsigned key: the order-preserving bits of v
-// (-0 taken as +0, which compares equal to it), then j, then i, each index in
-// 16 bits (n <= 65536).  No candidate (j = 0x7FFFFFFF) is the largest key.
-// NaN never becomes a candidate: a scan takes only values below f32::MAX.
-__device__ inline uint64_t cand_key(const Cand& c)
-{
-    if (c.j == 0x7FFFFFFF) return ~0ull;
-    const float v = c.v == 0.0f ? 0.0f : c.v;
-    return ((uint64_t)fkey(__float_as_uint(v)) << 32) | ((uint64_t)(unsigned)c.j << 16) | (unsigned)c.i;
+    v = v == 0.0f ? 0.0f : v;
+    return ((uint64_t)fkey(__float_as_uint(v)) << 32) | ((uint64_t)(unsigned)j << 16) | (unsigned)i;
 }
 
 __global__ void dense_from_upper_kernel(const float* __restrict__ up, int n, int ld, float* __restrict__ D,
@@ -266,9 +254,9 @@ struct MergeState {
 // decide kernel, are scanned as they are.
 __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D, int n, int ld,
                                                            const MergeState* __restrict__ st,
-                                                           Cand* __restrict__ part, int rule)
+                                                           uint64_t* __restrict__ part, int rule)
 {
-    __shared__ Cand wbest[4];
+    __shared__ uint64_t wbest[4];
     const int s = st->s;
     if (s >= n - 1) return;  // every merge found: a spare step of the graph
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -287,7 +275,7 @@ __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D
         mdij = st->dij;
     }
     const float kFltMax = 3.402823466e+38f, kInf = __builtin_inff();
-    Cand best{kFltMax, 0x7FFFFFFF, 0x7FFFFFFF};
+    uint64_t best = ~0ull;  // cand_key order: (value, j = row, i = column)
     for (int r = wid; r < order; r += W) {
         float* row = D + (int64_t)r * ld;
         const bool patch = s > 0 && r != mi_ && !(moved && r == mj_);
@@ -353,41 +341,23 @@ __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D
                 const float x = row[c_tail];
                 if (x < v) { v = x; bi = c_tail; }
             }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const float ov = __shfl_xor(v, o);
-                const int ob = __shfl_xor(bi, o);
-                if (ov < v || (ov == v && ob < bi)) {
-                    v = ov;
-                    bi = ob;
-                }
-            }
+            // the lane's candidate of this row joins its running minimum; the
+            // wave's lanes are combined once, after the last row
+            if (bi != 0x7FFFFFFF) best = min(best, cand_key(v, r, bi));
             // the patched prefix entries, under the scan's rule: a value below
-            // f32::MAX, first index on ties
-            if (pc0 >= 0 && pc0 < r && pv0 < kFltMax && (pv0 < v || (pv0 == v && pc0 < bi))) {
-                v = pv0;
-                bi = pc0;
-            }
-            if (pc1 >= 0 && pc1 < r && pv1 < kFltMax && (pv1 < v || (pv1 == v && pc1 < bi))) {
-                v = pv1;
-                bi = pc1;
-            }
-            const Cand c{v, r, bi};
-            if (bi != 0x7FFFFFFF && better(c, best)) best = c;
+            // f32::MAX (the same in every lane)
+            if (pc0 >= 0 && pc0 < r && pv0 < kFltMax) best = min(best, cand_key(pv0, r, pc0));
+            if (pc1 >= 0 && pc1 < r && pv1 < kFltMax) best = min(best, cand_key(pv1, r, pc1));
         }
         if (patch && lane == 0) {
             row[pc0] = pv0;
             if (pc1 >= 0) row[pc1] = pv1;
         }
     }
+    best = wave_min_u64(best);
     if (lane == 0) wbest[w] = best;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        Cand b = wbest[0];
-        for (int k = 1; k < 4; ++k)
-            if (better(wbest[k], b)) b = wbest[k];
-        part[blockIdx.x] = b;
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = min(min(wbest[0], wbest[1]), min(wbest[2], wbest[3]));
 }
 
 // The closest pair over the work-groups' candidates (merge s): records it,
@@ -397,7 +367,7 @@ __global__ __launch_bounds__(256) void cluster_step_kernel(float* __restrict__ D
 // i and j, so the threads never touch each other's data.
 __global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict__ D, int n, int ld,
                                                               MergeState* __restrict__ st,
-                                                              const Cand* __restrict__ part, int n_part,
+                                                              const uint64_t* __restrict__ part, int n_part,
                                                               unsigned* __restrict__ sizes, int rule,
                                                               unsigned* __restrict__ mi,
                                                               unsigned* __restrict__ mj,
@@ -409,7 +379,7 @@ __global__ __launch_bounds__(1024) void cluster_decide_kernel(float* __restrict_
     // the minimum key over the candidates: one wave reduction each, then the
     // 16 wave minima from LDS in every thread
     uint64_t key = ~0ull;
-    for (int k = threadIdx.x; k < n_part; k += 1024) key = min(key, cand_key(part[k]));
+    for (int k = threadIdx.x; k < n_part; k += 1024) key = min(key, part[k]);
     key = wave_min_u64(key);
     if ((threadIdx.x & 63) == 0) wkey[threadIdx.x >> 6] = key;
     __syncthreads();
@@ -564,12 +534,12 @@ void launch_cluster_step(float* D, int n, int ld, void* state, void* part, int n
                          int rule, unsigned* mi, unsigned* mj, float* md, hipStream_t s)
 {
     auto* st = static_cast<MergeState*>(state);
-    auto* cand = static_cast<Cand*>(part);
+    auto* cand = static_cast<uint64_t*>(part);
     cluster_step_kernel<<<n_part, 256, 0, s>>>(D, n, ld, st, cand, rule);
     cluster_decide_kernel<<<1, 1024, 0, s>>>(D, n, ld, st, cand, n_part, sizes, rule, mi, mj, md);
 }
 
 size_t cluster_state_bytes() { return sizeof(MergeState); }
-size_t cluster_candidate_bytes() { return sizeof(Cand); }
+size_t cluster_candidate_bytes() { return sizeof(uint64_t); }
 
 }  // namespace bs
