@@ -264,8 +264,8 @@ __device__ inline void mean_partial_block(const double* __restrict__ xyz,
 
 // Stage 2: coordinate v of r by one thread, in a fixed order (no atomics;
 // the single-ensemble and batched kernels share it, so both give the same
-// r): eight running sums over the slices s = q mod 8, so that eight loads
-// are in flight at a time, then added pairwise.
+// r): eight running sums over the slices s = q mod 8, fed by up to 32 loads
+// in flight at a time, then added pairwise.
 __device__ inline void mean_final_coord(const double* __restrict__ partial, int n_slices, int S,
                                         int L, int v, double* __restrict__ ref)
 {
@@ -273,6 +273,13 @@ __device__ inline void mean_final_coord(const double* __restrict__ partial, int 
     const double* p = partial + v;
     const int64_t ld = 3 * (int64_t)L;
     int s = 0;
+    for (; s + 32 <= n_slices; s += 32) {  // 32 loads in flight, added in the same order
+        double x[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) x[u] = p[(s + u) * ld];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) acc[u & 7] += x[u];
+    }
     for (; s + 8 <= n_slices; s += 8)
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[q] += p[(s + q) * ld];
