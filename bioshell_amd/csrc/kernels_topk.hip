@@ -26,6 +26,7 @@ constexpr int kMergeMax = 8192;
 constexpr int kChunkThreads = 256;
 constexpr int kChunkWaves = kChunkThreads / 64;
 constexpr int kMinChunk = 4096;
+constexpr int kCachedKeys = kMinChunk / kChunkThreads;  // keys per thread kept in registers
 
 __device__ inline uint32_t key_of(float v)
 {
@@ -76,13 +77,31 @@ __global__ __launch_bounds__(kChunkThreads) void topk_chunk_kernel(const float* 
     const float* row = scores + (int64_t)q * n + base;
     unsigned long long* out = cand + ((int64_t)q * gridDim.x + c) * k;
 
+    // the thread's first kCachedKeys keys (elements tid, tid + 256, ...) are
+    // read once and kept in registers for every pass; a longer chunk reads
+    // the rest from memory each time
+    uint32_t keys[kCachedKeys];
+#pragma unroll
+    for (int j = 0; j < kCachedKeys; ++j) {
+        const int e = j * kChunkThreads + tid;
+        keys[j] = e < len ? key_of(row[e]) : 0xFFFFFFFFu;
+    }
+    auto key_at = [&](int j, int e) { return j < kCachedKeys ? keys[j] : key_of(row[e]); };
+    const int iters = (len + kChunkThreads - 1) / kChunkThreads;  // <= 128 (chunk <= 32768)
+
     uint32_t prefix = 0, pmask = 0, need = (uint32_t)kc;
     for (int pass = 0; pass < 3; ++pass) {
         const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
         const int nb = pass < 2 ? 2048 : 1024;
         for (int b = tid; b < kChunkWaves * 2048; b += kChunkThreads) (&hist[0][0])[b] = 0;
         __syncthreads();
-        for (int e = tid; e < len; e += kChunkThreads) {
+#pragma unroll
+        for (int j = 0; j < kCachedKeys; ++j) {
+            const uint32_t key = keys[j];
+            if (j * kChunkThreads + tid < len && (key & pmask) == prefix)
+                atomicAdd(&hist[wv][(key >> shift) & (nb - 1)], 1u);
+        }
+        for (int e = kCachedKeys * kChunkThreads + tid; e < len; e += kChunkThreads) {
             const uint32_t key = key_of(row[e]);
             if ((key & pmask) == prefix) atomicAdd(&hist[wv][(key >> shift) & (nb - 1)], 1u);
         }
@@ -115,10 +134,9 @@ __global__ __launch_bounds__(kChunkThreads) void topk_chunk_kernel(const float* 
     }
     const uint32_t T = prefix;  // the kc-th smallest key; `need` copies of it are taken
     // rank the keys equal to T in index order: counts per (iteration, wave), one scan
-    const int iters = (len + kChunkThreads - 1) / kChunkThreads;  // <= 128 (chunk <= 32768)
     for (int j = 0; j < iters; ++j) {
         const int e = j * kChunkThreads + tid;
-        const bool eq = e < len && key_of(row[e]) == T;
+        const bool eq = e < len && key_at(j, e) == T;
         const uint64_t m = __ballot(eq);
         if (lane == 0) eq_cnt[j * kChunkWaves + wv] = (uint32_t)__popcll(m);
     }
@@ -136,7 +154,7 @@ __global__ __launch_bounds__(kChunkThreads) void topk_chunk_kernel(const float* 
     __syncthreads();
     for (int j = 0; j < iters; ++j) {
         const int e = j * kChunkThreads + tid;
-        const uint32_t key = e < len ? key_of(row[e]) : 0xFFFFFFFFu;
+        const uint32_t key = e < len ? key_at(j, e) : 0xFFFFFFFFu;
         const bool lt = e < len && key < T;
         const bool eq = e < len && key == T;
         const uint64_t m = __ballot(eq);
