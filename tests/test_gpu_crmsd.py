@@ -98,6 +98,27 @@ def test_ragged_shapes(gpu_ctx, n, L):
         assert err.max() <= tol, (method.name, err.max())
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_random_shapes_and_scales(gpu_ctx, seed):
+    """Seeded random ensembles: n up to 400, L up to 700, residual noise from
+    0.01 to 8 A, coordinates scaled by up to 20 (large residuals go to FP64
+    re-evaluation); every method against the oracle."""
+    rng = np.random.default_rng(1000 + seed)
+    n, L = int(rng.integers(2, 401)), int(rng.integers(1, 701))
+    lo = float(rng.uniform(0.01, 2.0))
+    x = B.generate_ensemble(2000 + seed, n, L, sigma=(lo, lo + float(rng.uniform(0.0, 6.0))),
+                            near_fraction=float(rng.uniform(0.0, 0.2)))
+    x = x * float(rng.choice([1.0, 1.0, 5.0, 20.0]))
+    want = O.all_vs_all(x, nthreads=8)
+    print(f"seed {seed}: n={n} L={L}", flush=True)
+    for method in ALL:
+        err = np.abs(run(gpu_ctx, x, method) - want)
+        report(f"random {method.name}", err)
+        # FP32 plain: its error grows with the coordinate magnitude
+        tol = TOL[method] * (float(np.abs(x).max()) / 50.0 if method == B.Method.FP32_PLAIN else 1.0)
+        assert err.max() <= max(tol, TOL[method]), (method, n, L)
+
+
 def test_single_conformation_and_errors(gpu_ctx):
     x = B.generate_ensemble(1, 1, 10)
     assert B.crmsd_all_vs_all(x, ctx=gpu_ctx).upper.size == 0
