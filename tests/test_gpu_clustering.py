@@ -41,7 +41,8 @@ def random_upper(rng, n, kind):
 
 @pytest.mark.parametrize("linkage", C.LINKAGES)
 @pytest.mark.parametrize("n,kind", [(2, "uniform"), (3, "ties"), (17, "uniform"), (64, "ties"),
-                                    (65, "euclid"), (203, "uniform"), (150, "ties")])
+                                    (65, "euclid"), (203, "uniform"), (150, "ties"),
+                                    (1500, "ties"), (1100, "euclid")])
 def test_merges_bit_identical_to_oracle(gpu_ctx, linkage, n, kind):
     rng = np.random.default_rng(n * 7 + len(kind))
     up = random_upper(rng, n, kind)
