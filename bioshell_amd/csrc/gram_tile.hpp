@@ -171,6 +171,11 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
 
+    // the K loop runs at a raised wave priority and the FP64 epilogue at the
+    // normal one: with two work-groups per CU in different phases, the
+    // sequencer then favours the MFMA stream (measured 2 % faster than equal
+    // priorities, 0.6 % for the opposite choice)
+    __builtin_amdgcn_s_setprio(2);
     BS_ISSUE(0, 0)
     BS_ISSUE(1, 1)
     for (int kb = 0; kb < n_kblk; kb += 3) {
@@ -183,6 +188,7 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
 #undef BS_ISSUE
 #undef BS_STEP
 
+    __builtin_amdgcn_s_setprio(0);
     epilogue<64>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
 }
 
