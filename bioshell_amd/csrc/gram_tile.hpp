@@ -211,8 +211,10 @@ __device__ __forceinline__ void gram_tile32(const GramArgs& a, int tr, int tc, c
     for (int q = 0; q < 9; ++q)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
+    __builtin_amdgcn_s_setprio(2);  // as in gram_tile64: the K loop ahead of other waves' epilogues
     for (int kb = 0; kb < a.n_kblk; ++kb)
         mfma_block<MODE>(A + (int64_t)kb * kPanelBlockU4, B + (int64_t)kb * kPanelBlockU4, lane, acc);
+    __builtin_amdgcn_s_setprio(0);
     epilogue<32>(a, tr, tc, acc, lds, thread);
 }
 
