@@ -68,7 +68,7 @@ def measured_peaks() -> dict:
 
     from bioshell_amd import _native
 
-    f = _native.lib().bsi_peak_probe
+    f = _native.diag_lib().bsi_peak_probe
     f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     f.restype = ctypes.c_int
     out = {}
@@ -448,6 +448,38 @@ def next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x) -> dict:
     return out
 
 
+def host_info() -> dict:
+    """The host the CPU baseline runs on: logical CPUs (nproc), the CPUs this
+    process may run on (its affinity mask), the thread share the machine's
+    environment grants (OMP_NUM_THREADS, where set), and lscpu's model,
+    sockets, cores per socket and threads per core."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": int(os.environ["OMP_NUM_THREADS"]) if os.environ.get("OMP_NUM_THREADS", "").isdigit()
+            else None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads_default() -> int:
+    """Every CPU this process may use, capped by the thread share the
+    environment grants (OMP_NUM_THREADS) where one is set."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
 def cpu_baseline(x: np.ndarray, target_s: float, threads: int) -> dict:
     """Times the FP64 oracle on rows [0, R) with R sized for ~target_s seconds."""
     from oracle import oracle as O
@@ -481,7 +513,9 @@ def main() -> None:
     ap.add_argument("--n-atoms", type=int, default=300)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: every CPU of the affinity mask, capped by "
+                         "OMP_NUM_THREADS where set)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -500,6 +534,8 @@ def main() -> None:
                          "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
+    if args.cpu_threads <= 0:
+        args.cpu_threads = cpu_threads_default()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the all-vs-all path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
@@ -648,6 +684,7 @@ def main() -> None:
             "value": round(cb["value"], 1),
             "unit": "superpositions/s",
             "cores": args.cpu_threads,
+            "host": host_info(),
             "kind": "port",
             "sample": f"rows 0..{R - 1} of the same 10,000 x 300 ensemble ({cb['pairs']} pairs), "
                       f"FP64 QCP oracle, {cb['seconds']:.1f} s",

@@ -11,6 +11,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbioshell_gpu.so")
+DIAG_LIB_PATH = os.path.join(HERE, "libbioshell_diag.so")
 
 c_status = ctypes.c_int32
 c_dp = ctypes.POINTER(ctypes.c_double)
@@ -78,6 +79,7 @@ SIGNATURES = {
     "bs_context_last_stats": (c_status, [c_vp, ctypes.POINTER(RunStats)]),
     "bs_context_bytes_in_use": (c_size, [c_vp]),
     "bs_context_history": (c_status, [c_vp, ctypes.POINTER(RunStats), c_size, ctypes.POINTER(c_size)]),
+    "bs_default_context": (c_status, [ctypes.POINTER(c_vp)]),
     "bs_crmsd_pair_f64": (c_status, [c_vp, c_dp, c_dp, c_size, c_dp]),
     "bs_superpose_f64": (c_status, [c_vp, c_dp, c_dp, c_size, c_dp, c_dp, c_dp]),
     "bs_crmsd_all_vs_all": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_vp, ctypes.c_int]),
@@ -107,6 +109,8 @@ SIGNATURES = {
     "bs_server_wait": (c_status, [c_vp, ctypes.c_uint64]),
     "bs_server_cancel": (c_status, [c_vp, ctypes.c_uint64]),
     "bs_server_latency_ms": (c_status, [c_vp, ctypes.c_uint64, c_dp]),
+    "bs_server_release": (c_status, [c_vp, ctypes.c_uint64]),
+    "bs_server_ticket_count": (c_size, [c_vp]),
     "bs_superpose_ensemble": (c_status, [c_vp, c_dp, c_size, c_size, c_dp, c_size, c_dp, c_dp, c_dp, c_dp]),
     "bs_superpose_ensemble_device": (c_status, [c_vp, c_vp, c_size, c_size, c_vp, c_size, c_vp, c_vp, c_vp,
                                                 c_vp]),
@@ -145,3 +149,17 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+_diag = None
+
+
+def diag_lib() -> ctypes.CDLL:
+    """The diagnostics library (peak-rate probes, host builds of the device QCP
+    routines): used by the benchmark and the CPU tests, never by the package."""
+    global _diag
+    if _diag is None:
+        if not os.path.exists(DIAG_LIB_PATH):
+            raise ImportError(f"{DIAG_LIB_PATH} is missing: build the libraries first")
+        _diag = ctypes.CDLL(DIAG_LIB_PATH)
+    return _diag

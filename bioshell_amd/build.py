@@ -13,8 +13,12 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+DIAG = os.path.join(HERE, "diag")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbioshell_gpu.so")
+# peak-rate probes and host builds of the device QCP code, for the bench and
+# the CPU tests; kept out of the product library
+DIAG_LIB = os.path.join(HERE, "libbioshell_diag.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON_FLAGS = [
@@ -27,15 +31,16 @@ COMMON_FLAGS = [
 ]
 
 
-def sources() -> list[str]:
+def sources(where: str = CSRC) -> list[str]:
     return sorted(
-        os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp"))
+        os.path.join(where, f) for f in os.listdir(where) if f.endswith((".hip", ".cpp"))
     )
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *COMMON_FLAGS, "-c", src, "-o", obj]
+    sub = "diag_" if os.path.dirname(src) == DIAG else ""
+    obj = os.path.join(OBJ, sub + os.path.basename(src) + ".o")
+    cmd = [HIPCC, *COMMON_FLAGS, "-I", CSRC, "-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -48,19 +53,25 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None) -> str:
-    """Compiles all sources and links the library; returns its path."""
-    os.makedirs(OBJ, exist_ok=True)
-    srcs = sources()
-    jobs = jobs or min(8, os.cpu_count() or 1)
-    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+def _link(objs: list[str], lib: str) -> None:
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp, *objs, "-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build(jobs: int | None = None) -> str:
+    """Compiles all sources and links the product library (and the
+    diagnostics library); returns the product library's path."""
+    os.makedirs(OBJ, exist_ok=True)
+    srcs, diag = sources(), sources(DIAG)
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs + diag))
+    _link(objs[:len(srcs)], LIB)
+    _link(objs[len(srcs):], DIAG_LIB)
     return LIB
 
 

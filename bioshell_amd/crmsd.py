@@ -79,7 +79,11 @@ def _dp(a: np.ndarray):
 class Context:
     """One device, one stream, workspaces (bs_context)."""
 
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, _handle=None):
+        self._owned = _handle is None
+        if _handle is not None:
+            self._h = _handle
+            return
         h = ctypes.c_void_p()
         check(_native.lib().bs_context_create(device, hbm_budget_bytes, ctypes.byref(h)))
         self._h = h
@@ -91,8 +95,10 @@ class Context:
         return self._h
 
     def close(self) -> None:
+        """Destroys the context (the library's default context is never destroyed)."""
         if getattr(self, "_h", None) is not None:
-            _native.lib().bs_context_destroy(self._h)
+            if self._owned:
+                _native.lib().bs_context_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -146,9 +152,14 @@ _default: Context | None = None
 
 
 def default_context() -> Context:
+    """The library's process-wide default context (bs_default_context: device 0,
+    no budget, kept until the process ends) -- the one the C entry points use
+    when given a null context, as the context-free Rust functions do."""
     global _default
-    if _default is None:
-        _default = Context(0)
+    if _default is None or _default._h is None:
+        h = ctypes.c_void_p()
+        check(_native.lib().bs_default_context(ctypes.byref(h)))
+        _default = Context(_handle=h)
     return _default
 
 
@@ -381,10 +392,24 @@ class Library:
 
 # --------------------------------------------------------------- serving
 class Ticket:
-    """A submitted request; keeps its buffers alive until it completes."""
+    """A submitted request.  It owns its result buffers; the server copied the
+    queries at submit.  Dropping the ticket releases it (bs_server_release): a
+    request still running then finishes without writing the buffers."""
 
-    def __init__(self, server: "Server", tid: int, q: np.ndarray, idx: np.ndarray, val: np.ndarray):
-        self.server, self.id, self._q, self.idx, self.rmsd = server, tid, q, idx, val
+    def __init__(self, server: "Server", tid: int, idx: np.ndarray, val: np.ndarray):
+        self.server, self.id, self.idx, self.rmsd = server, tid, idx, val
+
+    def release(self) -> None:
+        """Gives the ticket up; its id is invalid afterwards."""
+        if self.id is not None and self.server._h is not None:
+            _native.lib().bs_server_release(self.server._h, self.id)
+        self.id = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
     def poll(self) -> bool:
         st = _native.lib().bs_server_poll(self.server.handle, self.id)
@@ -444,4 +469,8 @@ class Server:
         check(_native.lib().bs_server_submit(self.handle, lib_index, _dp(q), q.shape[0], k,
                                              idx.ctypes.data_as(_native.c_u32p),
                                              val.ctypes.data_as(_native.c_fp), ctypes.byref(t)))
-        return Ticket(self, t.value, q, idx, val)
+        return Ticket(self, t.value, idx, val)
+
+    def ticket_count(self) -> int:
+        """Ticket records the server holds (bs_server_ticket_count)."""
+        return _native.lib().bs_server_ticket_count(self.handle)

@@ -3,12 +3,33 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <string>
 
 #include "engine.hpp"
 
 namespace bs {
 const char* last_error_cstr();
+
+// The process-wide default context: device 0, no HBM budget, created by the
+// first call that needs it and kept until the process ends (never destroyed,
+// so no call can race its teardown; the driver frees its device memory at
+// exit).  A failed creation is remembered and reported to every later call.
+bs_context* resolve_context(bs_context* ctx)
+{
+    if (ctx) return ctx;
+    static std::once_flag once;
+    static bs_context* dflt = nullptr;
+    static bs_status dflt_status = BS_OK;
+    static std::string dflt_error;
+    std::call_once(once, [] {
+        dflt_status = bs_context_create(0, 0, &dflt);
+        if (dflt_status != BS_OK) dflt_error = bs_last_error();
+    });
+    if (dflt_status != BS_OK) fail(dflt_status, "default context: %s", dflt_error.c_str());
+    return dflt;
 }
+}  // namespace bs
 
 using namespace bs;
 
@@ -58,7 +79,6 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         ctx->device = device;
         ctx->n_cu = prop.multiProcessorCount;
         ctx->gsc.max_wg = 2 * ctx->n_cu;
-        if (const char* e = std::getenv("BIOSHELL_GRAM_VARIANT")) ctx->gsc.variant = std::atoi(e);
         ctx->budget.set_cap(hbm_budget_bytes);
         BS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         BS_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
@@ -145,13 +165,22 @@ bs_status bs_context_history(const bs_context* ctx, bs_run_stats* out, size_t ma
 
 size_t bs_context_bytes_in_use(const bs_context* ctx) { return ctx ? ctx->budget.in_use() : 0; }
 
+bs_status bs_default_context(bs_context** out)
+{
+    return guard([&] {
+        BS_REQUIRE(out != nullptr, "out is null");
+        *out = nullptr;
+        *out = resolve_context(nullptr);
+    });
+}
+
 bs_status bs_crmsd_pair_f64(bs_context* ctx, const double* a_xyz, const double* b_xyz,
                             size_t n_atoms, double* out)
 {
     return guard([&] {
-        check_ctx(ctx);
         BS_REQUIRE(a_xyz && b_xyz && out, "null pointer argument");
         check_shape(2, n_atoms);
+        ctx = resolve_context(ctx);
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         hipStream_t s = ctx->stream;
@@ -169,9 +198,9 @@ bs_status bs_superpose_f64(bs_context* ctx, const double* a_xyz, const double* b
                            size_t n_atoms, double* rot9, double* trans3, double* rmsd)
 {
     return guard([&] {
-        check_ctx(ctx);
         BS_REQUIRE(a_xyz && b_xyz && rot9 && trans3, "null pointer argument");
         check_shape(2, n_atoms);
+        ctx = resolve_context(ctx);
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         hipStream_t s = ctx->stream;
@@ -195,12 +224,12 @@ bs_status bs_superpose_ensemble(bs_context* ctx, const double* xyz, size_t n_con
                                 double* rmsd, double* superposed)
 {
     return guard([&] {
-        check_ctx(ctx);
         check_shape(n_conf, n_atoms);
         BS_REQUIRE(n_conf >= 1, "n_conf must be >= 1");
         BS_REQUIRE(xyz, "null coordinates");
         BS_REQUIRE(ref_xyz || ref_index < n_conf, "ref_index %zu out of range (%zu conformations)",
                    ref_index, n_conf);
+        ctx = resolve_context(ctx);
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         hipStream_t s = ctx->stream;
@@ -273,11 +302,11 @@ bs_status bs_crmsd_all_vs_all(bs_context* ctx, const double* xyz, size_t n_conf,
                               bs_method method, void* out_upper, int out_f64)
 {
     return guard([&] {
-        check_ctx(ctx);
         check_shape(n_conf, n_atoms);
         BS_REQUIRE(valid_method(method), "unknown method %d", (int)method);
         if (n_conf < 2) return;
         BS_REQUIRE(xyz && out_upper, "null pointer argument");
+        ctx = resolve_context(ctx);
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         hipStream_t s = ctx->stream;

@@ -425,6 +425,7 @@ bs_status bs_crmsd_batch(bs_context* ctx, const double* const* xyz, const size_t
                          float* const* out_upper)
 {
     return guard([&] {
+        ctx = resolve_context(ctx);
         check_batch_args(ctx, n_conf, n_atoms, n_ens, method);
         BS_REQUIRE(n_ens == 0 || (xyz && out_upper), "null pointer argument");
         std::lock_guard<std::mutex> lk(ctx->mu);

@@ -107,6 +107,7 @@ bs_status bs_rmsd_row_stats(bs_context* ctx, const float* upper, size_t n, float
                             float* row_max)
 {
     return guard([&] {
+        ctx = resolve_context(ctx);
         check_n(ctx, n);
         BS_REQUIRE((n < 2 || upper) && row_min && row_max, "null pointer argument");
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -131,6 +132,7 @@ bs_status bs_rmsd_members_row_max(bs_context* ctx, const float* upper, size_t n,
                                   const uint32_t* members, size_t m, float* row_max)
 {
     return guard([&] {
+        ctx = resolve_context(ctx);
         check_n(ctx, n);
         BS_REQUIRE(members && row_max && (n < 2 || upper), "null pointer argument");
         BS_REQUIRE(m >= 1 && m <= n, "m must be in [1, n]");
@@ -174,6 +176,7 @@ bs_status bs_hierarchical_clustering(bs_context* ctx, const float* upper, size_t
                                      float* merge_d)
 {
     return guard([&] {
+        ctx = resolve_context(ctx);
         check_n(ctx, n);
         BS_REQUIRE(linkage >= BS_SINGLE_LINK && linkage <= BS_WARDS_METHOD, "unknown linkage %d",
                    (int)linkage);

@@ -186,10 +186,7 @@ void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScra
     g.max_wg = sc.max_wg > 0 ? sc.max_wg : 512;
 
     if (rec) rec->mark({kBPrep1, kBMain0}, s);  // the bitmap clear counts as preparation
-    if (sc.variant == 1)
-        launch_gram_pipe(g, A.mode, s);
-    else
-        launch_gram(g, A.mode, s);
+    launch_gram(g, A.mode, s);
     if (rec) rec->mark({kBMain1, kBFallback0}, s);
     BS_HIP(hipGetLastError());
     launch_fallback(A.xyz, A.cent, B.xyz, B.cent, A.L, flags,

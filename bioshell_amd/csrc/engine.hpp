@@ -12,6 +12,10 @@
 
 namespace bs {
 
+// `ctx`, or for a null `ctx` the process-wide default context (device 0, no
+// HBM budget, created on first use and kept until the process ends).
+bs_context* resolve_context(bs_context* ctx);
+
 // Conformations of one atom count, prepared on the device for the Gram path:
 // the raw coordinates they came from (borrowed: they must outlive every run
 // that reads this set, for the FP64 re-evaluation), centroids, rotations
@@ -92,7 +96,6 @@ struct GramScratch {
     TileOrder orders[kOrders];
     int next_order = 0;
     int max_wg = 0;               // persistent Gram work-groups (2 per CU)
-    int variant = 0;              // 0: one tile per work-group, 1: software-pipelined, persistent
     uint32_t* h_order = nullptr;  // pinned staging of a tile order being uploaded
     size_t h_order_cap = 0;
     hipEvent_t order_done = nullptr;

@@ -100,8 +100,6 @@ struct GramArgs {
 int build_tile_order(int tiles_r, int tiles_c, bool triangle, int super, uint32_t* out);
 
 void launch_gram(const GramArgs& a, GramMode mode, hipStream_t s);
-// The software-pipelined variant (one persistent work-group per CU).
-void launch_gram_pipe(const GramArgs& a, GramMode mode, hipStream_t s);
 void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
 // FP64 re-evaluation of the outputs whose bit is set in `bits` (n_out bits),
@@ -125,6 +123,10 @@ void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s);
 // sorted by (value, index): idx and val get n_q x k each.  scratch holds
 // topk_scratch_bytes(n_q, n_lib, k).
 size_t topk_scratch_bytes(int n_q, int n_lib, int k);
+// Whether the two-stage selection handles k of n_lib (k <= 1024, and the
+// chunks' candidates fit the merge): launch_topk fails otherwise.  Monotone in
+// k for a given n_lib.
+bool topk_supported(int n_lib, int k);
 void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, float* val, void* scratch,
                  hipStream_t s);
 

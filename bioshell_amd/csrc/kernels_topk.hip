@@ -221,16 +221,24 @@ size_t topk_scratch_bytes(int n_q, int n_lib, int k)
     return (size_t)n_q * n_chunks * k * 8;
 }
 
+bool topk_supported(int n_lib, int k)
+{
+    if (k <= 0 || k > kTopkMax || k > n_lib) return false;
+    const int chunk = topk_chunk_len(n_lib, k);
+    const int n_chunks = (n_lib + chunk - 1) / chunk;
+    return chunk <= 128 * kChunkThreads && n_chunks * k <= kMergeMax;
+}
+
 void launch_topk(const float* scores, int n_q, int n_lib, int k, uint32_t* idx, float* val, void* scratch,
                  hipStream_t s)
 {
     if (n_q <= 0) return;
     if (k <= 0 || k > kTopkMax || k > n_lib) fail(BS_ERR_INVALID_ARGUMENT, "top-k: k = %d outside [1, min(%d, %d)]",
                                                   k, kTopkMax, n_lib);
+    if (!topk_supported(n_lib, k))
+        fail(BS_ERR_UNSUPPORTED, "top-k: %d library members with k = %d exceed the selection's range", n_lib, k);
     const int chunk = topk_chunk_len(n_lib, k);
     const int n_chunks = (n_lib + chunk - 1) / chunk;
-    if (chunk > 128 * kChunkThreads || n_chunks * k > kMergeMax)
-        fail(BS_ERR_UNSUPPORTED, "top-k: %d library members with k = %d exceed the selection's range", n_lib, k);
     auto* cand = static_cast<unsigned long long*>(scratch);
     topk_chunk_kernel<<<dim3(n_chunks, n_q), kChunkThreads, 0, s>>>(scores, n_lib, k, chunk, cand);
     topk_merge_kernel<<<n_q, 1024, 0, s>>>(cand, n_chunks * k, k, idx, val);

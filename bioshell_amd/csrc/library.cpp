@@ -86,7 +86,6 @@ void enqueue_query(bs_library* lib, QueryWorkspace& ws, const double* q_host, in
     run.out_f64 = false;
     run.out_ld = n_lib;
     ws.gsc.max_wg = lib->ctx->gsc.max_wg;
-    ws.gsc.variant = lib->ctx->gsc.variant;
     run_gram(view_of(ws.prep), view_of(lib->P), run, ws.gsc, s, rec);
     if (k > 0) {
         uint32_t* idx = static_cast<uint32_t*>(ws.topk.ensure(topk_bytes(n_q, n_lib, k), "top-k"));

@@ -375,6 +375,100 @@ BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj
     return shifted_finish(s, st);
 }
 
+// ---------------------------------------------------------------------------
+// The same largest root through the Schur complement of K' = [[0, a^T], [a, B]]:
+//     det(mu I - K') = det(mu I - B) * f(mu),  f(mu) = mu - a^T (mu I - B)^{-1} a.
+// When S = -B is positive definite -- the usual case, since B is dominated by
+// the reference's B(R0) = 2 R0 - 2 tr(R0) I, whose eigenvalues are minus twice
+// the sums of pairs of R0's eigenvalues -- every eigenvalue of B is negative,
+// the eigenvalues of K' interlace B's, and the largest root mu* (>= 0, as K'
+// has a zero diagonal entry) is the only root of f in [0, inf).  There f is
+// increasing and concave (its second derivative is -2 a^T (S + mu I)^{-3} a),
+// so Newton from mu = 0 approaches mu* from below, and
+//     g0(mu) = a^T (S + mu I)^{-1} a >= mu*   for every mu <= mu*
+// bounds it from above: after a step from mu, mu* lies in [mu', g0(mu)].  The
+// step solves (S + mu I) v = a by the adjugate (v = adj(S + mu I) a / det),
+// f'(mu) = 1 + v.v, so the step and the bracket cost one 3x3 solve.  With
+// residuals e small against the reference, mu* is itself a small correction
+// to H (it is the gain of the residual rotation between the two
+// conformations) and one step leaves a bracket of relative width
+// |v|^2 ~ mu*/|S|, far below the tolerance for chains of a few hundred atoms;
+// short chains take a second step.  A pair whose S is not positive definite,
+// or whose bracket stays too wide, is solved with the quartic above.
+struct SecularPair {
+    double b00, b11, b22, b01, b02, b12;  // B
+    double a0, a1, a2;                    // a
+    double H;
+    double lo, hi;  // mu* in [lo, hi]
+    bool pd;        // S = -B positive definite
+};
+
+// One Newton step of f from mu = st.lo (st.lo = 0 at the start): returns
+// whether S + mu I was found positive definite (else the bracket is void).
+BS_HD bool secular_step(SecularPair& st)
+{
+    const double m = st.lo;
+    // S + m I = -(B - m I); adj(-X) = adj(X) for 3x3, det(-X) = -det(X)
+    const double c00 = st.b00 - m, c11 = st.b11 - m, c22 = st.b22 - m;
+    const double A00 = c11 * c22 - st.b12 * st.b12;
+    const double A11 = c00 * c22 - st.b02 * st.b02;
+    const double A22 = c00 * c11 - st.b01 * st.b01;
+    const double A01 = st.b02 * st.b12 - st.b01 * c22;
+    const double A02 = st.b01 * st.b12 - st.b02 * c11;
+    const double A12 = st.b01 * st.b02 - c00 * st.b12;
+    const double det = -(c00 * A00 + st.b01 * A01 + st.b02 * A02);  // det(S + m I)
+    const double w0 = A00 * st.a0 + A01 * st.a1 + A02 * st.a2;
+    const double w1 = A01 * st.a0 + A11 * st.a1 + A12 * st.a2;
+    const double w2 = A02 * st.a0 + A12 * st.a1 + A22 * st.a2;
+    const double aw = st.a0 * w0 + st.a1 * w1 + st.a2 * w2;  // det * g0(m)
+    const double ww = w0 * w0 + w1 * w1 + w2 * w2;           // det^2 * |v|^2
+    // Sylvester: leading minors of S + m I are -c00, A22, det
+    const bool pd = (c00 < 0.0) & (A22 > 0.0) & (det > 0.0);
+    // new = m - (m - g0) / (1 + |v|^2) = (m ww + aw det) / (det^2 + ww), and
+    // g0(m) = aw / det: both quotients from one reciprocal of det (det^2 + ww)
+    const double den = det * det + ww;
+    const double r = newton_div(1.0, det * den);
+    st.lo = (m * ww + aw * det) * (det * r);
+    st.hi = aw * (den * r);
+    return pd;
+}
+
+// B, a and H of one pair (as shifted_poly), then the first step from 0.
+BS_HD SecularPair secular_first(const double* E, const double* ci, const double* cj)
+{
+    SecularPair st;
+    const double trE = E[0] + E[4] + E[8];
+    st.b00 = ci[0] + cj[0] + 2.0 * (E[0] - trE);
+    st.b11 = ci[1] + cj[1] + 2.0 * (E[4] - trE);
+    st.b22 = ci[2] + cj[2] + 2.0 * (E[8] - trE);
+    st.b01 = ci[3] + cj[3] + (E[1] + E[3]);
+    st.b02 = ci[4] + cj[4] + (E[2] + E[6]);
+    st.b12 = ci[5] + cj[5] + (E[5] + E[7]);
+    st.a0 = (cj[6] - ci[6]) + (E[5] - E[7]);
+    st.a1 = (cj[7] - ci[7]) + (E[6] - E[2]);
+    st.a2 = (cj[8] - ci[8]) + (E[1] - E[3]);
+    st.H = (ci[9] + cj[9]) - trE;
+    st.lo = 0.0;
+    st.pd = secular_step(st);
+    return st;
+}
+
+// The result of a bracket: mu* is taken at its midpoint; converged when S is
+// positive definite and the half-width h, an error bound of H - mu, moves the
+// RMSD by at most tol: h <= tol L RMSD, i.e. h^2 <= tol2L * (L RMSD^2) with
+// tol2L = tol^2 L and L RMSD^2 = 2 (H - mu).  A NaN or negative h fails.
+BS_HD ShiftedOut secular_out(const SecularPair& st, double tol2L)
+{
+    ShiftedOut o;
+    const double mu = 0.5 * (st.lo + st.hi), half = 0.5 * (st.hi - st.lo);
+    const double d = st.H - mu;
+    o.rmsd2_L = 2.0 * (d > 0.0 ? d : 0.0);
+    o.H = st.H;
+    o.iterations = 1;
+    o.converged = st.pd & (half >= 0.0) & (half * half <= tol2L * o.rmsd2_L);
+    return o;
+}
+
 // Two pairs sharing a column, solved in lockstep: coefficients, the first
 // Newton step and its verification are straight-line code for both, so the
 // two dependency chains interleave; a pair whose verification fails (rare)

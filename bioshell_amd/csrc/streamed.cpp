@@ -48,7 +48,6 @@ bs_status bs_crmsd_all_vs_all_streamed(bs_context* ctx, const double* xyz, size_
                                        float* out_upper)
 {
     return guard([&] {
-        BS_REQUIRE(ctx != nullptr, "context is null");
         BS_REQUIRE(n_atoms >= 1 && n_atoms <= (1u << 20), "n_atoms must be in [1, 2^20]");
         BS_REQUIRE(n_conf <= (1u << 21), "n_conf too large (%zu)", n_conf);
         BS_REQUIRE(method >= BS_FP64_PLAIN && method <= BS_BF16X3_GRAM, "unknown method %d",
@@ -57,6 +56,7 @@ bs_status bs_crmsd_all_vs_all_streamed(bs_context* ctx, const double* xyz, size_
                    "tile must be a multiple of 64 in [64, 16384] (got %zu)", tile);
         if (n_conf < 2) return;
         BS_REQUIRE(xyz && out_upper, "null pointer argument");
+        ctx = resolve_context(ctx);
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         hipStream_t s = ctx->stream, cs = ctx->copy_stream;

@@ -1,11 +1,10 @@
-// kernels_probe.hip -- peak-rate probes: the dense BF16 MFMA rate and the FP64
+// peak_probe.hip -- peak-rate probes: the dense BF16 MFMA rate and the FP64
 // FMA rate this GPU sustains, as reference points for the achieved rates the
-// benchmark reports next to the published peaks (not part of the public ABI).
+// benchmark reports next to the published peaks.  Built into the diagnostics
+// library (libbioshell_diag.so), not the product library.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-
-#include "common.hpp"
 
 namespace bs {
 namespace {

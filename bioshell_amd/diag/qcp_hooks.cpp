@@ -1,0 +1,61 @@
+// qcp_hooks.cpp -- host builds of the device QCP routines, exported for CPU
+// tests of the algebra.  Built into the diagnostics library
+// (libbioshell_diag.so), not the product library.
+#include "prep_device.hpp"
+#include "qcp.hpp"
+
+extern "C" {
+
+// out[0] = 2 (H - mu) (= L * RMSD^2), out[1] = H, out[2] = iterations,
+// out[3] = converged (1/0).
+void bsi_qcp_shifted(const double* E, const double* ci, const double* cj, double* out)
+{
+    const bs::ShiftedOut o = bs::qcp_shifted(E, ci, cj);
+    out[0] = o.rmsd2_L;
+    out[1] = o.H;
+    out[2] = o.iterations;
+    out[3] = o.converged ? 1.0 : 0.0;
+}
+
+double bsi_qcp_rmsd_f64(const double* M, double Ga, double Gb, int n)
+{
+    return bs::qcp_rmsd<double>(M, Ga, Gb, n);
+}
+
+// Rotation (row-major) that moves b onto a, from M = sum b a^T, as the
+// preparation kernels compute it.
+void bsi_rotation_fast(const double* M, double* R) { bs::rotation_fast(M, R); }
+
+float bsi_qcp_rmsd_f32(const float* M, float Ga, float Gb, int n)
+{
+    return bs::qcp_rmsd<float>(M, Ga, Gb, n);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// The secular-form solve with its fallbacks (as the Gram epilogue runs it):
+// out[0] = 2 (H - mu), out[1] = H, out[2] = steps taken (1, 2; 0 when the
+// quartic solved it), out[3] = converged (1/0).
+void bsi_qcp_secular(const double* E, const double* ci, const double* cj, double tol2L, double* out)
+{
+    bs::SecularPair st = bs::secular_first(E, ci, cj);
+    bs::ShiftedOut o = bs::secular_out(st, tol2L);
+    double steps = 1.0;
+    if (!o.converged && st.pd) {
+        st.pd = bs::secular_step(st);
+        o = bs::secular_out(st, tol2L);
+        steps = 2.0;
+    }
+    if (!o.converged) {
+        o = bs::qcp_shifted(E, ci, cj);
+        steps = 0.0;
+    }
+    out[0] = o.rmsd2_L;
+    out[1] = o.H;
+    out[2] = steps;
+    out[3] = o.converged ? 1.0 : 0.0;
+}
+
+}  // extern "C"

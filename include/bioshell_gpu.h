@@ -22,8 +22,17 @@
  *    (substructures.rs:19) come back as BS_ERR_INVALID_ARGUMENT.
  *  - Host buffers are borrowed for the duration of a synchronous call.
  *    Device memory belongs to the context / library / server that allocated it.
- *  - A bs_context is used by one caller at a time; distinct contexts may be
- *    used from different threads.
+ *  - Calls on one bs_context are serialised by the context (each takes its
+ *    lock); distinct contexts run independently.
+ *  - Default context.  The synchronous entry points that take host buffers
+ *    (bs_crmsd_pair_f64, bs_superpose_f64, bs_superpose_ensemble,
+ *    bs_crmsd_all_vs_all, bs_crmsd_all_vs_all_streamed, bs_crmsd_batch,
+ *    bs_rmsd_row_stats, bs_rmsd_members_row_max, bs_hierarchical_clustering)
+ *    accept ctx == NULL: they then run on the process-wide default context
+ *    (device 0, no HBM budget), created by the first such call and kept until
+ *    the process ends.  This is what the context-free Rust functions
+ *    crmsd(a, b) and crmsd_all_vs_all(models, method) bind to.  If it cannot
+ *    be created (no GPU), every call that needs it returns that error.
  */
 #ifndef BIOSHELL_GPU_H
 #define BIOSHELL_GPU_H
@@ -107,13 +116,18 @@ bs_status bs_context_history(const bs_context* ctx, bs_run_stats* out, size_t ma
                              size_t* n_out);
 /* Device bytes currently allocated by the context (workspaces included). */
 size_t bs_context_bytes_in_use(const bs_context* ctx);
+/* The process-wide default context that ctx == NULL selects (created here if
+ * no call has needed it yet).  Never destroy it; it lives until the process
+ * ends.  For its stream, stats and history, and for the device entry points. */
+bs_status bs_default_context(bs_context** out);
 
 /* ------------------------------------------------------- single pair */
 
 /* cRMSD of two conformations of n_atoms each, FP64 on the GPU.
  * Stands in for the intended `bioshell_pdb::calc::crmsd(a: &[T], b: &[T]) -> f64`
  * (bioshell-pdb/Cargo.toml:27), generic over HasCartesians like
- * SubstructureAxis::from_3d_points (bioshell-pdb/src/calc/substructures.rs:17). */
+ * SubstructureAxis::from_3d_points (bioshell-pdb/src/calc/substructures.rs:17).
+ * ctx may be NULL (the default context): the Rust crmsd(a, b) takes none. */
 bs_status bs_crmsd_pair_f64(bs_context* ctx, const double* a_xyz, const double* b_xyz,
                             size_t n_atoms, double* out);
 
@@ -366,8 +380,13 @@ bs_status bs_query_all(bs_context* ctx, bs_library* lib, const double* q_xyz, si
 bs_status bs_server_create(bs_context* ctx, bs_library* const* libs, size_t n_libs,
                            size_t hbm_budget_bytes, int max_inflight, bs_server** out);
 void bs_server_destroy(bs_server* srv);
-/* Submits n_q queries against library lib_index; q_xyz must stay valid until
- * the ticket completes.  Results go to idx/rmsd (n_q*k each). */
+/* Submits n_q queries against library lib_index.  The queries are copied
+ * before the call returns; results go to idx/rmsd (n_q*k each), which must
+ * stay valid until the ticket completes or is released.  k beyond the top-k
+ * selection's range for that library returns BS_ERR_UNSUPPORTED here, so a
+ * request never fails later because of the requests it was merged with.
+ * The server keeps each ticket's record (status, latency) until
+ * bs_server_release; bs_server_destroy finishes every queued request. */
 bs_status bs_server_submit(bs_server* srv, size_t lib_index, const double* q_xyz, size_t n_q,
                            size_t k, uint32_t* idx, float* rmsd, uint64_t* ticket);
 /* 0 = done, BS_ERR_NOT_READY = still running, other = the request's error. */
@@ -378,6 +397,13 @@ bs_status bs_server_wait(bs_server* srv, uint64_t ticket);
 bs_status bs_server_cancel(bs_server* srv, uint64_t ticket);
 /* Latency of a completed ticket (submit .. result visible on host), in ms. */
 bs_status bs_server_latency_ms(bs_server* srv, uint64_t ticket, double* ms);
+/* Drops a ticket's record.  A finished ticket goes at once.  A queued one is
+ * withdrawn.  A running one finishes without writing idx/rmsd (they may be
+ * freed as soon as this returns) and its record goes when it ends.  The
+ * ticket id is invalid afterwards. */
+bs_status bs_server_release(bs_server* srv, uint64_t ticket);
+/* Ticket records the server holds (submitted and not yet released). */
+size_t bs_server_ticket_count(bs_server* srv);
 
 /* ------------------------------------------------------- synthetic input */
 

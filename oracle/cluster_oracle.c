@@ -26,8 +26,12 @@
  * Pinned by tests/test_clustering_oracle.py: the reference's clustering tests,
  * MST heights (single), scipy heights (single, complete; average, centroid and
  * median on a chain where no cluster moves), the textbook first merge of
- * Ward's rule.  Ward's later merges (which depend on the sizes[j] quirk) are
- * parity partial: only a hand-traced case checks them.
+ * Ward's rule, and hand-traced six-element cases for average, centroid and
+ * Ward linkage in which a merged cluster is moved and then merged again (the
+ * sizes[j] argument and the sizes left in place decide those heights).
+ * or_hclust_variant() plants the two plausible mistakes -- sizes moved with
+ * the row, sizes[k] as the third argument -- so that the test can show the
+ * hand-traced cases reject them.
  */
 #include <float.h>
 #include <math.h>
@@ -60,8 +64,11 @@ static float or_rule(int rule, long si, long sj, long sk, float dij, float dik, 
 }
 
 /* upper: packed upper triangle (i < j, row-major) of n elements.  Writes the
- * n - 1 merges.  Returns 0, or -1 on allocation failure. */
-int or_hclust(const float* upper, long n, int rule, int* mi, int* mj, float* md)
+ * n - 1 merges.  Returns 0, or -1 on allocation failure.
+ * variant 0 is the reference's loop; 1 moves sizes[last] to j with the row,
+ * 2 passes sizes[k] as the rule's third argument (planted mistakes). */
+int or_hclust_variant(const float* upper, long n, int rule, int variant, int* mi, int* mj,
+                      float* md)
 {
     if (n < 2) return 0;
     float* m = (float*)malloc(sizeof(float) * (size_t)n * (size_t)n);
@@ -101,7 +108,8 @@ int or_hclust(const float* upper, long n, int rule, int* mi, int* mj, float* md)
         /* update_distances(i, j, rule, new_index = i) */
         const long si = sizes[bi], sj = sizes[bj];
         for (long k = 0; k < order; ++k)
-            res[k] = or_rule(rule, si, sj, sizes[bj], dij, M(bi, k), M(bj, k));
+            res[k] = or_rule(rule, si, sj, variant == 2 ? sizes[k] : sizes[bj], dij, M(bi, k),
+                             M(bj, k));
         for (long k = 0; k < order; ++k) {
             M(bi, k) = res[k];
             M(k, bi) = res[k];
@@ -118,6 +126,7 @@ int or_hclust(const float* upper, long n, int rule, int* mi, int* mj, float* md)
                 M(order, c) = t;
             }
             for (long r = 0; r < order; ++r) M(r, bj) = M(r, order);
+            if (variant == 1) sizes[bj] = sizes[order];
         } else {
             order -= 1;
         }
@@ -127,6 +136,11 @@ int or_hclust(const float* upper, long n, int rule, int* mi, int* mj, float* md)
     free(sizes);
     free(res);
     return 0;
+}
+
+int or_hclust(const float* upper, long n, int rule, int* mi, int* mj, float* md)
+{
+    return or_hclust_variant(upper, n, rule, 0, mi, mj, md);
 }
 
 /* Row minimum and maximum over j != i (retrieve_outliers scans for the minimum

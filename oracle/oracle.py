@@ -176,17 +176,19 @@ def tri_index(n: int, i, j):
 LINKAGES = ("single", "complete", "average", "median", "centroid", "wards")
 
 
-def hclust(upper, n: int, linkage: str):
+def hclust(upper, n: int, linkage: str, planted_mistake: int = 0):
     """Merge sequence of BioShell's agglomerative clustering (cluster_oracle.c):
     arrays i, j (current matrix indices, i < j) and the merging distance of
-    each of the n - 1 steps."""
+    each of the n - 1 steps.  ``planted_mistake`` 1 moves sizes with the row
+    on a move, 2 passes sizes[k] to the rule (for tests that show a check
+    rejects them); 0 is the reference's behaviour."""
     up = np.ascontiguousarray(upper, dtype=np.float32)
     if up.shape != (n * (n - 1) // 2,):
         raise ValueError("packed triangle has the wrong length")
     L = lib()
-    fn = L.or_hclust
+    fn = L.or_hclust_variant
     fn.restype = ctypes.c_int
-    fn.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_long, ctypes.c_int,
+    fn.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_long, ctypes.c_int, ctypes.c_int,
                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                    ctypes.POINTER(ctypes.c_float)]
     m = max(n - 1, 0)
@@ -195,7 +197,7 @@ def hclust(upper, n: int, linkage: str):
     md = np.zeros(m, dtype=np.float32)
     fp = ctypes.POINTER(ctypes.c_float)
     ip = ctypes.POINTER(ctypes.c_int)
-    if fn(up.ctypes.data_as(fp), n, LINKAGES.index(linkage), mi.ctypes.data_as(ip),
+    if fn(up.ctypes.data_as(fp), n, LINKAGES.index(linkage), int(planted_mistake), mi.ctypes.data_as(ip),
           mj.ctypes.data_as(ip), md.ctypes.data_as(fp)) != 0:
         raise MemoryError("or_hclust")
     return mi, mj, md

@@ -31,6 +31,26 @@ def test_every_declared_symbol_is_exported():
     assert sorted(_native.SIGNATURES) == syms
 
 
+@pytest.mark.skipif(B.device_count() > 0, reason="checks the no-device error path")
+def test_default_context_without_device_is_an_error():
+    """A null context selects the process-wide default context; without a GPU
+    every call that needs it reports why it could not be created."""
+    L = _native.lib()
+    a = np.zeros((4, 3))
+    out = ctypes.c_double()
+    for _ in range(2):  # the failed creation is remembered, not retried
+        st = L.bs_crmsd_pair_f64(None, a.ctypes.data_as(_native.c_dp), a.ctypes.data_as(_native.c_dp), 4,
+                                 ctypes.byref(out))
+        assert st == -2 and b"default context" in L.bs_last_error()
+    h = ctypes.c_void_p()
+    assert L.bs_default_context(ctypes.byref(h)) == -2 and not h.value
+    x = np.zeros((1, 5, 3))
+    up = np.zeros(0, np.float32)
+    # nothing to compute: returns before the default context is needed
+    assert L.bs_crmsd_all_vs_all(None, x.ctypes.data_as(_native.c_dp), 1, 5, 3,
+                                 up.ctypes.data_as(ctypes.c_void_p), 0) == 0
+
+
 def test_version_and_device_count():
     assert "gfx950" in B.version()
     assert B.device_count() >= 0
@@ -49,6 +69,8 @@ def test_null_and_invalid_arguments_return_status():
     assert L.bs_context_create(0, 0, None) == -1
     assert b"null" in L.bs_last_error()
     assert L.bs_crmsd_pair_f64(None, None, None, 3, None) == -1
+    assert b"null pointer" in L.bs_last_error()
+    assert L.bs_context_synchronize(None) == -1  # the asynchronous side has no default
     assert b"context is null" in L.bs_last_error()
     assert L.bs_context_last_stats(None, None) == -1
     out = np.empty((2, 3, 3))
@@ -117,7 +139,7 @@ def test_generator_golden(golden_config0):
 
 # ------------------------------------------------------------ host QCP builds
 def _hook(name, res, args):
-    fn = getattr(_native.lib(), name)
+    fn = getattr(_native.diag_lib(), name)
     fn.restype = res
     fn.argtypes = args
     return fn
@@ -169,6 +191,65 @@ def test_host_build_of_shifted_qcp_matches_oracle():
             fn(E.ctypes.data_as(dp), cs[i].ctypes.data_as(dp), cs[j].ctypes.data_as(dp), out.ctypes.data_as(dp))
             assert out[3] == 1.0
             assert np.sqrt(out[0] / 50) == pytest.approx(O.crmsd(x[i], x[j]), abs=1e-8)
+
+
+def _kabsch(P, Q):
+    """Rotation R minimising |P - Q R^T| for centred P, Q (numpy SVD)."""
+    U, _, Vt = np.linalg.svd(Q.T @ P)
+    d = np.sign(np.linalg.det(Vt.T @ U.T))
+    return Vt.T @ np.diag([1.0, 1.0, d]) @ U.T
+
+
+def _shift_consts(e, r):
+    C, R0, g = r.T @ e, r.T @ r, (e * e).sum()
+    t0, tc = np.trace(R0), np.trace(C)
+    return np.array([
+        R0[0, 0] - t0 + 2 * (C[0, 0] - tc), R0[1, 1] - t0 + 2 * (C[1, 1] - tc),
+        R0[2, 2] - t0 + 2 * (C[2, 2] - tc), R0[0, 1] + C[0, 1] + C[1, 0],
+        R0[0, 2] + C[0, 2] + C[2, 0], R0[1, 2] + C[1, 2] + C[2, 1],
+        C[1, 2] - C[2, 1], C[2, 0] - C[0, 2], C[0, 1] - C[1, 0], g / 2, np.sqrt(g)])
+
+
+@pytest.mark.parametrize("L,sigma,aligned", [(300, (0.5, 4.0), True), (76, (1.5, 1.5), True), (30, (0.5, 3.0), True),
+                                             (12, (4.0, 8.0), True), (3, (0.5, 3.0), True), (50, (0.5, 4.0), False)])
+def test_host_build_of_secular_qcp_matches_oracle(L, sigma, aligned):
+    """The Gram epilogue's solve: Newton on the secular form of the QCP
+    polynomial from mu = 0, a second step when the bracket is wide, the
+    quartic when S = -B is not positive definite.  Exact residual Grams (FP64)
+    against the oracle's cRMSD; with conformations fitted onto their mean (the
+    device's frame) long chains converge in one step."""
+    from oracle import oracle as O
+
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn = _hook("bsi_qcp_secular", None, [dp, dp, dp, ctypes.c_double, dp])
+    n = 40
+    x = B.generate_ensemble(17 + L, n, L, sigma=sigma, near_fraction=0.1)
+    xc = np.array([a - a.mean(0) for a in x])
+    if aligned:
+        fit0 = np.array([c @ _kabsch(xc[0], c).T for c in xc])
+        ref = fit0.mean(0)
+        e = np.array([c @ _kabsch(ref, c).T for c in xc]) - ref
+    else:  # any reference is exact, only slower: noisy and unaligned
+        ref = xc[0] + np.random.default_rng(1).normal(size=xc[0].shape)
+        ref -= ref.mean(0)
+        e = xc - ref
+    cs = [np.ascontiguousarray(_shift_consts(ei, ref)) for ei in e]
+    tol2L = (2e-5 / 4) ** 2 * L
+    out = np.empty(4)
+    steps = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            E = np.ascontiguousarray((e[i].T @ e[j]).ravel())
+            fn(E.ctypes.data_as(dp), cs[i].ctypes.data_as(dp), cs[j].ctypes.data_as(dp), tol2L,
+               out.ctypes.data_as(dp))
+            steps.append(out[2])
+            want = O.crmsd(x[i], x[j])
+            if out[3] == 1.0:
+                assert np.sqrt(out[0] / L) == pytest.approx(want, abs=2e-5 / 4 + 1e-9), (i, j, out)
+    steps = np.array(steps)
+    print(f"L={L}: one step {np.mean(steps == 1):.3f}, two {np.mean(steps == 2):.3f}, quartic {np.mean(steps == 0):.3f}")
+    if aligned and L >= 76:
+        assert np.all(steps == 1)
 
 
 def test_host_build_of_fast_rotation():

@@ -175,31 +175,103 @@ def test_wards_first_merge_is_textbook():
 
 
 def test_oracle_keeps_reference_quirks():
-    """clust --median/--centroid map to average/wards (clust.rs:66-80); the
-    merge rule is called with sizes[j] for every k and sizes are not moved
-    with the last row -- checked by replaying a hand-traced 4-element case."""
+    """clust --median/--centroid map to average/wards (clust.rs:66-80)."""
     assert C.linkage_of_cli(median=True) == "average"
     assert C.linkage_of_cli(centroid=True) == "wards"
     assert C.linkage_of_cli() == "single"
-    # d: 0-1 = 1, 2-3 = 2, others 10; Ward's rule:
-    # step 1 merges (0,1) at 1: j=1 < last=3, so element 3 moves to index 1
-    # but sizes stay [2, 1, 1]; the new row 0 has
-    # d(0,k) = (1/(1+1+1)) * (1+1) * 10 + (1/3) * (1+1) * 10 - 1 * (1/3) * 1
-    D = np.full((4, 4), 10.0, np.float32)
-    np.fill_diagonal(D, 0)
-    D[0, 1] = D[1, 0] = 1.0
-    D[2, 3] = D[3, 2] = 2.0
-    up = np.array([D[i, j] for i in range(4) for j in range(i + 1, 4)], dtype=np.float32)
-    mi, mj, md = O.hclust(up, 4, "wards")
-    assert (int(mi[0]), int(mj[0]), float(md[0])) == (0, 1, 1.0)
-    assert (int(mi[1]), int(mj[1]), float(md[1])) == (1, 2, 2.0)
-    third = np.float32(1) / np.float32(3)
-    d02 = np.float32(third * np.float32(2) * np.float32(10) + third * np.float32(2) * np.float32(10)
-                     - np.float32(1) * third * np.float32(1))
-    # third merge: clusters {0,1} and {2,3}; its distance comes from the row
-    # updated with sizes[j] = 1 (the moved element kept the old size 1)
-    assert md[2] > 0 and np.isfinite(md[2])
-    assert d02 == pytest.approx(13.0, abs=1e-5)
+
+
+# Hand-traced agglomeration of six elements in which a merged cluster is moved
+# and then merged again, so that the sizes the reference keeps in place
+# (clustering_matrix.rs:47-53: rows swapped, columns copied, sizes untouched)
+# and the sizes[j] it passes as the rule's third argument
+# (clustering_matrix.rs:66) decide the later heights.
+F = np.float32
+
+
+def _average(si, sj, sk, dij, dik, djk):  # strategies/mod.rs average_link, f32
+    d = F(1.0) / F(si + sj)
+    return F(F(d * F(si)) * dik) + F(F(d * F(sj)) * djk)
+
+
+def _centroid(si, sj, sk, dij, dik, djk):  # strategies/mod.rs centroid_link, f32
+    d = F(1.0) / F(si + sj)
+    return (F(F(d * F(si)) * dik) + F(F(d * F(sj)) * djk)) - F(F(F(F(si) * F(sj)) * d) * d) * dij
+
+
+def _wards(si, sj, sk, dij, dik, djk):  # strategies/mod.rs wards_method, f32
+    d = F(1.0) / F(si + sj + sk)
+    return (F(F(d * F(si + sk)) * dik) + F(F(d * F(sj + sk)) * djk)) - F(F(sk) * d) * dij
+
+
+def _six():
+    """d(4,5) = 1 merges first (no move: 5 is last); d(0,1) = 2 next, after which
+    the pair {4,5} at index 4 moves to index 1 while sizes[1] stays 1."""
+    D = np.zeros((6, 6), np.float32)
+    for (a, b), v in {(0, 1): 2, (4, 5): 1, (2, 4): 3, (2, 5): 3, (0, 2): 10, (1, 2): 10,
+                      (0, 3): 30, (1, 3): 30, (0, 4): 8, (0, 5): 12, (1, 4): 8, (1, 5): 12,
+                      (2, 3): 16, (3, 4): 20, (3, 5): 24}.items():
+        D[a, b] = D[b, a] = v
+    return D
+
+
+def _hand_trace(rule):
+    """Expected (i, j, height) of the five merges, step by step."""
+    D = _six()
+    P = {k: rule(1, 1, 1, D[4, 5], D[4, k], D[5, k]) for k in (0, 1, 2, 3)}  # step 1: {4,5} at 4
+    # step 2: (0, 1) at 2; row 0 becomes A = {0,1}; sizes[1] = 1 is the third argument
+    A = {2: rule(1, 1, 1, D[0, 1], D[0, 2], D[1, 2]), 3: rule(1, 1, 1, D[0, 1], D[0, 3], D[1, 3]),
+         "P": rule(1, 1, 1, D[0, 1], P[0], P[1])}
+    # {4,5} moves to index 1; sizes stay [2, 1, 1, 1 | 2]: index 1 still says 1
+    # step 3: (1, 2) = ({4,5}, {2}) with size_i = 1 (not 2), size_j = 1, sizes[j] = 1
+    dP2 = P[2]
+    Q = {"A": rule(1, 1, 1, dP2, A["P"], A[2]), 3: rule(1, 1, 1, dP2, P[3], D[2, 3])}
+    # {3} (index 3, the last) moves to index 2; sizes[1] = 1 + 1 = 2 (the cluster holds 3)
+    # step 4: (0, 1) = (A, Q), size_i = 2, size_j = 2, sizes[j] = 2
+    dAQ = Q["A"]
+    AQ3 = rule(2, 2, 2, dAQ, A[3], Q[3])
+    # {3} moves to index 1; step 5: (0, 1) at AQ3
+    return [(4, 5, F(1)), (0, 1, F(2)), (1, 2, dP2), (0, 1, dAQ), (0, 1, AQ3)]
+
+
+@pytest.mark.parametrize("linkage,rule", [("average", _average), ("centroid", _centroid), ("wards", _wards)])
+def test_moved_cluster_heights_hand_traced(linkage, rule):
+    D = _six()
+    up = D[np.triu_indices(6, 1)]
+    want = _hand_trace(rule)
+    mi, mj, md = O.hclust(up, 6, linkage)
+    got = list(zip(mi.tolist(), mj.tolist(), md.tolist()))
+    assert [(i, j) for i, j, _ in got] == [(i, j) for i, j, _ in want]
+    assert np.array_equal(md.view(np.uint32), np.array([h for _, _, h in want], np.float32).view(np.uint32)), \
+        (got, want)
+    # and the values are the ones worked out on paper for average linkage
+    if linkage == "average":
+        assert [float(h) for _, _, h in want] == [1.0, 2.0, 3.0, 10.0, 24.5]
+    if linkage == "centroid":
+        assert [float(h) for _, _, h in want] == [1.0, 2.0, 2.75, 8.6875, 21.671875]
+    # planted mistakes: sizes moved with the row changes the last height for
+    # every size-weighted rule; sizes[k] as the third argument changes Ward's
+    moved = O.hclust(up, 6, linkage, planted_mistake=1)[2]
+    assert not np.array_equal(moved, md)
+    by_k = O.hclust(up, 6, linkage, planted_mistake=2)[2]
+    assert np.array_equal(by_k, md) == (linkage != "wards")
+
+
+def test_wards_moved_cluster_hand_computed():
+    """Ward's rule on the same six elements, worked out on paper (exact
+    fractions; f32 agrees to a few ulps): d(P,k) = (2(d4k + d5k) - 1)/3, so
+    P = {0: 13, 1: 13, 2: 11/3, 3: 29}; A = {2: 38/3, 3: 118/3, P: 50/3};
+    step 3 with the stale size 1: Q = {A: (2(50/3 + 38/3) - 11/3)/3 = 55/3,
+    3: (2(29 + 16) - 11/3)/3 = 259/9}; step 4 with sizes 2, 2, 2:
+    (4 * 118/3 + 4 * 259/9 - 2 * 55/3)/6 = 1061/27."""
+    D = _six()
+    md = O.hclust(D[np.triu_indices(6, 1)], 6, "wards")[2].astype(np.float64)
+    assert md == pytest.approx([1.0, 2.0, 11 / 3, 55 / 3, 1061 / 27], rel=2e-7)
+    # with the moved cluster's true size 2 at step 3, Q = {A: 215/12, 3: 173/6}
+    # and step 4 (sizes 2, 3, 3) gives 5/8 * 118/3 + 6/8 * 173/6 - 3/8 * 215/12
+    moved = O.hclust(D[np.triu_indices(6, 1)], 6, "wards", planted_mistake=1)[2].astype(np.float64)
+    assert moved[4] == pytest.approx(5 / 8 * 118 / 3 + 6 / 8 * 173 / 6 - 3 / 8 * 215 / 12, rel=2e-7)
+    assert abs(moved[4] - 1061 / 27) > 0.1
 
 
 def test_row_stats_oracle():

@@ -53,6 +53,21 @@ def test_merges_bit_identical_to_oracle(gpu_ctx, linkage, n, kind):
     assert np.array_equal(got[2].view(np.uint32), want[2].view(np.uint32)), "merge distance bits"
 
 
+@pytest.mark.parametrize("linkage", ["average", "centroid", "wards"])
+def test_moved_cluster_hand_trace_on_device(gpu_ctx, linkage):
+    """The hand-traced six-element case (a merged cluster moved, then merged
+    again with the size left at its new index) gives the traced heights on the
+    device too."""
+    from tests.test_clustering_oracle import _average, _centroid, _hand_trace, _six, _wards
+
+    rule = {"average": _average, "centroid": _centroid, "wards": _wards}[linkage]
+    want = _hand_trace(rule)
+    D = _six()
+    mi, mj, md = device_merges(gpu_ctx, D[np.triu_indices(6, 1)], 6, linkage)
+    assert list(zip(mi.tolist(), mj.tolist())) == [(i, j) for i, j, _ in want]
+    assert np.array_equal(md.view(np.uint32), np.array([h for _, _, h in want], np.float32).view(np.uint32))
+
+
 @pytest.mark.parametrize("linkage", ["single", "average", "wards"])
 def test_rmsd_matrix_clustering_and_device_entry(gpu_ctx, linkage):
     """An all-vs-all RMSD matrix: host triangle and HBM triangle give the same

@@ -226,10 +226,48 @@ def test_device_pointer_entry(gpu_ctx):
     assert np.abs(got - want).max() <= 1e-4
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-def test_gram_kernel_variants(variant, monkeypatch):
-    """Both Gram kernels (one tile per work-group; persistent software-pipelined)."""
-    monkeypatch.setenv("BIOSHELL_GRAM_VARIANT", str(variant))
+def test_default_context_through_the_c_abi():
+    """A null context runs on the process-wide default context -- the binding
+    of the context-free Rust crmsd(a, b) and crmsd_all_vs_all(models, method);
+    the Python default_context() is that same context."""
+    import ctypes
+
+    L = B._native.lib()
+    x = B.generate_ensemble(21, 30, 40, sigma=(0.5, 3.0))
+    out = ctypes.c_double()
+    assert L.bs_crmsd_pair_f64(None, x[0].ctypes.data_as(B._native.c_dp), x[1].ctypes.data_as(B._native.c_dp),
+                               40, ctypes.byref(out)) == 0
+    assert abs(out.value - O.crmsd(x[0], x[1])) <= 1e-9
+    up = np.empty(30 * 29 // 2, np.float32)
+    assert L.bs_crmsd_all_vs_all(None, x.ctypes.data_as(B._native.c_dp), 30, 40, int(B.Method.BF16X3_GRAM),
+                                 up.ctypes.data_as(ctypes.c_void_p), 0) == 0
+    assert np.abs(up.astype(np.float64) - O.all_vs_all(x)).max() <= TOL[B.Method.BF16X3_GRAM]
+    h1, h2 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.bs_default_context(ctypes.byref(h1)) == 0 and L.bs_default_context(ctypes.byref(h2)) == 0
+    assert h1.value and h1.value == h2.value == B.default_context().handle.value
+    # the stats of the null-context call are on the default context
+    assert B.default_context().last_stats()["pairs"] == 30 * 29 // 2
+    # closing the Python wrapper does not destroy the library's context
+    B.default_context().close()
+    assert B.crmsd(x[0], x[1]) == out.value
+    # threads share it safely (each call takes the context's lock)
+    import threading
+    res = [None] * 4
+
+    def work(t):
+        res[t] = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM).upper
+
+    th = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for r in res:
+        assert np.array_equal(r, up)
+
+
+def test_gram_kernel_outputs_f32_and_f64():
+    """The Gram kernel with f32 and f64 output, both split modes."""
     x = B.generate_ensemble(8, 700, 120, sigma=(0.5, 4.0), near_fraction=0.05)
     want = O.all_vs_all(x, nthreads=8)
     with B.Context(0) as ctx:
@@ -237,7 +275,7 @@ def test_gram_kernel_variants(variant, monkeypatch):
             for f64 in (False, True):
                 got = B.crmsd_all_vs_all(x, method, out_f64=f64, ctx=ctx).upper.astype(np.float64)
                 err = np.abs(got - want)
-                report(f"variant {variant} {method.name} f64={f64}", err)
+                report(f"{method.name} f64={f64}", err)
                 assert err.max() <= TOL[method]
 
 

@@ -192,3 +192,59 @@ def test_server_budget_and_cancel(gpu_ctx):
             t.wait()
     srv.close()
     lib.close()
+
+
+def test_server_tickets_are_released_and_bounded(gpu_ctx):
+    """Ticket records go when the caller releases them (dropping a Python
+    Ticket does): many requests leave the server's table bounded.  A ticket
+    dropped while queued or running never writes its (freed) buffers."""
+    import gc
+
+    lx, qq = make_bucket(55, 500, 16, 32)
+    lib = B.Library(lx, ctx=gpu_ctx)
+    srv = B.Server([lib], max_inflight=2)
+    want = lib.query_topk(qq[:2], 5)
+    for r in range(300):
+        t = srv.submit(0, qq[:2], 5)
+        if r % 3 == 0:
+            idx, val = t.wait()
+            assert np.array_equal(idx, want[0])
+            assert t.latency_ms() > 0.0
+        del t  # finished or not: released
+        if r % 50 == 0:
+            gc.collect()
+    gc.collect()
+    assert srv.ticket_count() <= 4, srv.ticket_count()
+    # explicit release of a waited ticket, then its id is unknown
+    t = srv.submit(0, qq, 5)
+    t.wait()
+    tid = t.id
+    t.release()
+    assert B._native.lib().bs_server_poll(srv.handle, tid) == -1
+    # a queue of tickets dropped at once, then the server still answers
+    burst = [srv.submit(0, qq, 5) for _ in range(40)]
+    del burst
+    gc.collect()
+    idx, val = srv.submit(0, qq[:2], 5).wait()
+    assert np.array_equal(idx, want[0]) and np.array_equal(val, want[1])
+    srv.close()
+    lib.close()
+
+
+def test_server_rejects_k_beyond_selection_at_submit(gpu_ctx):
+    """k the top-k selection cannot take for a library fails at submit, so it
+    can never take requests merged with it down; small k beside it still run."""
+    lx = B.generate_ensemble(66, 300_000, 8, sigma=(0.5, 3.0))
+    lib = B.Library(lx, ctx=gpu_ctx)
+    srv = B.Server([lib], max_inflight=1)
+    q = lx[:2]
+    small = [srv.submit(0, q, 10) for _ in range(3)]
+    with pytest.raises(B.BioshellGpuError) as e:
+        srv.submit(0, q, 1024)  # 300k members at k = 1024: beyond the merge's 8192 candidates
+    assert e.value.status == -5
+    alone = lib.query_topk(q, 10)
+    for t in small:
+        idx, val = t.wait()
+        assert np.array_equal(idx, alone[0]) and np.array_equal(val, alone[1])
+    srv.close()
+    lib.close()
