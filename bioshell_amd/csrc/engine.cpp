@@ -13,14 +13,54 @@ bool is_gram(bs_method m) { return m == BS_FP32_GRAM || m == BS_BF16X3_GRAM; }
 
 static constexpr int kTileRows = 64;  // pairs per work-group tile side
 
-size_t prepared_bytes(int n, int L)
+PreparedSizes prepared_sizes(int n, int L)
 {
     const size_t n_pad = (size_t)round_up(n, kTileRows);
     const size_t n_kblk = (size_t)(L + kAtomBlock - 1) / kAtomBlock;
     const size_t slices = (size_t)(ref_sample_count(n) + kMeanSliceConfs - 1) / kMeanSliceConfs;
-    return (size_t)n * 3 * 8 + (size_t)n * 9 * 8 + (size_t)(3 * L + 6) * 8 +
-           slices * 3 * L * 8 + n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16 +
-           (size_t)kNumConsts * n_pad * 8;
+    PreparedSizes z;
+    z.cent = (size_t)n * 3 * 8;
+    z.rot = (size_t)n * 9 * 8;
+    z.ref = (size_t)(3 * L + 6) * 8;
+    z.partial = slices * 3 * L * 8;
+    z.panels = n_pad / kConfBlock * n_kblk * kPanelBlockU4 * 16;
+    z.consts = (size_t)kNumConsts * n_pad * 8;
+    return z;
+}
+
+size_t prepared_bytes(int n, int L)
+{
+    const PreparedSizes z = prepared_sizes(n, L);
+    return z.cent + z.rot + z.ref + z.partial + z.panels + z.consts;
+}
+
+static PrepView prep_view(const PreparedPtrs& p, const double* d_xyz, int n, int L, GramMode mode)
+{
+    PrepView v;
+    v.panels = p.panels;
+    v.consts = p.consts;
+    v.xyz = d_xyz;
+    v.cent = p.cent;
+    v.ld = round_up(n, kTileRows);
+    v.n = n;
+    v.L = L;
+    v.n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
+    v.mode = mode;
+    return v;
+}
+
+PrepView prepare_into(const PreparedPtrs& p, const double* d_xyz, int n, int L, GramMode mode,
+                      const double* fixed_ref, hipStream_t s)
+{
+    const int n_pad = round_up(n, kTileRows), n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
+    const double* ref = fixed_ref;  // a fixed reference is read in place
+    if (!fixed_ref) {
+        launch_reference(d_xyz, n, L, p.rot, p.cent, p.partial, p.ref, s);
+        ref = p.ref;
+    }
+    launch_fit_pack(d_xyz, ref, n, n_pad, L, n_kblk, mode, p.rot, p.cent, p.panels, p.consts, n_pad, s);
+    BS_HIP(hipGetLastError());
+    return prep_view(p, d_xyz, n, L, mode);
 }
 
 void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
@@ -32,23 +72,17 @@ void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
     P.n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
     P.mode = mode;
     P.xyz = d_xyz;
-    double* cent = static_cast<double*>(P.cent.ensure((size_t)n * 3 * 8, "centroids"));
-    double* rot = static_cast<double*>(P.rot.ensure((size_t)n * 9 * 8, "rotations"));
-    void* panels = P.panels.ensure((size_t)P.n_pad / kConfBlock * P.n_kblk * kPanelBlockU4 * 16,
-                                   "operand panels");
-    double* consts = static_cast<double*>(P.consts.ensure((size_t)kNumConsts * P.n_pad * 8, "constants"));
-
-    const double* ref = fixed_ref;  // a fixed reference is read in place
+    const PreparedSizes z = prepared_sizes(n, L);
+    PreparedPtrs p;
+    p.cent = static_cast<double*>(P.cent.ensure(z.cent, "centroids"));
+    p.rot = static_cast<double*>(P.rot.ensure(z.rot, "rotations"));
+    p.panels = P.panels.ensure(z.panels, "operand panels");
+    p.consts = static_cast<double*>(P.consts.ensure(z.consts, "constants"));
     if (!fixed_ref) {
-        double* own = static_cast<double*>(P.ref.ensure((size_t)(3 * L + 6) * 8, "reference"));
-        const size_t slices = (size_t)(ref_sample_count(n) + kMeanSliceConfs - 1) / kMeanSliceConfs;
-        double* partial = static_cast<double*>(
-            P.partial.ensure(slices * 3 * L * 8, "reference partial sums"));
-        launch_reference(d_xyz, n, L, rot, cent, partial, own, s);
-        ref = own;
+        p.ref = static_cast<double*>(P.ref.ensure(z.ref, "reference"));
+        p.partial = static_cast<double*>(P.partial.ensure(z.partial, "reference partial sums"));
     }
-    launch_fit_pack(d_xyz, ref, n, P.n_pad, L, P.n_kblk, mode, rot, cent, panels, consts, P.n_pad, s);
-    BS_HIP(hipGetLastError());
+    (void)prepare_into(p, d_xyz, n, L, mode, fixed_ref, s);
 }
 
 GramThresholds gram_thresholds(GramMode mode, int L, int n_kblk)
@@ -138,23 +172,16 @@ PrepView view_of(const Prepared& P, int first, int count)
     return v;
 }
 
-void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScratch& sc,
-              hipStream_t s, RunRecord* rec)
+void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, const uint32_t* order,
+                   int n_tiles, uint32_t* flags, int list_cap, int max_wg, hipStream_t s, RunRecord* rec)
 {
-    BS_REQUIRE(A.L == B.L && A.mode == B.mode, "prepared sets disagree on atom count or mode");
-    const int tiles_r = (A.n + kTileRows - 1) / kTileRows, tiles_c = (B.n + kTileRows - 1) / kTileRows;
-    BS_REQUIRE(tiles_r < 65536 && tiles_c < 65536, "too many conformations for one launch");
-    int n_tiles = 0;
-    const uint32_t* order = sc.tile_order(tiles_r, tiles_c, run.triangle, s, &n_tiles);
     // Flagged outputs are marked in a bitmap indexed by output position (one
     // bit per pair: 6.25 MB for 50M pairs), so there is no capacity limit and
     // no host round trip; after the Gram kernel the set bits are queued (the
     // word after the bitmap counts them) and re-evaluated one wave each.
     const uint64_t n_out =
         run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * (uint64_t)run.out_ld;
-    const uint64_t n_words = ((n_out + 31) / 32 + 3) / 4 * 4;
-    uint32_t* flags = static_cast<uint32_t*>(sc.flags.ensure(
-        (size_t)(n_words + 4) * 4 + (size_t)kFlagListCap * 8, "flag bitmap and queue"));
+    const uint64_t n_words = flag_words(n_out);
     uint32_t* count = flags + n_words;
     int64_t* list = reinterpret_cast<int64_t*>(flags + n_words + 4);
     launch_zero_words(flags, (int64_t)n_words + 4, s);
@@ -183,15 +210,14 @@ void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScra
     g.acc_coef = th.acc_coef;
     g.tau = th.tau;
     g.rep_g_max = th.rep_g_max;
-    g.max_wg = sc.max_wg > 0 ? sc.max_wg : 512;
+    g.max_wg = max_wg > 0 ? max_wg : 512;
 
     if (rec) rec->mark({kBPrep1, kBMain0}, s);  // the bitmap clear counts as preparation
     launch_gram(g, A.mode, s);
     if (rec) rec->mark({kBMain1, kBFallback0}, s);
     BS_HIP(hipGetLastError());
-    launch_fallback(A.xyz, A.cent, B.xyz, B.cent, A.L, flags,
-                    (int64_t)n_out, g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, count, list,
-                    kFlagListCap, s);
+    launch_fallback(A.xyz, A.cent, B.xyz, B.cent, A.L, flags, (int64_t)n_out, g.triangle, A.n, run.out_ld,
+                    run.d_out, g.out_f64, count, list, list_cap, s);
     BS_HIP(hipGetLastError());
     if (rec && rec->h_count)
         BS_HIP(hipMemcpyAsync(rec->h_count, count, 4, hipMemcpyDeviceToHost, s));
@@ -203,6 +229,21 @@ void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScra
             rec->mark({kBFallback1}, s);
         rec->gram = true;
     }
+}
+
+void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScratch& sc,
+              hipStream_t s, RunRecord* rec)
+{
+    BS_REQUIRE(A.L == B.L && A.mode == B.mode, "prepared sets disagree on atom count or mode");
+    const int tiles_r = (A.n + kTileRows - 1) / kTileRows, tiles_c = (B.n + kTileRows - 1) / kTileRows;
+    BS_REQUIRE(tiles_r < 65536 && tiles_c < 65536, "too many conformations for one launch");
+    int n_tiles = 0;
+    const uint32_t* order = sc.tile_order(tiles_r, tiles_c, run.triangle, s, &n_tiles);
+    const uint64_t n_out =
+        run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * (uint64_t)run.out_ld;
+    uint32_t* flags = static_cast<uint32_t*>(
+        sc.flags.ensure(flag_region_bytes(n_out, kFlagListCap), "flag bitmap and queue"));
+    run_gram_with(A, B, run, order, n_tiles, flags, kFlagListCap, sc.max_wg, s, rec);
 }
 
 void GramScratch::reset()

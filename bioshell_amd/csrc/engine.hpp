@@ -59,6 +59,21 @@ PrepView view_of(const Prepared& P, int first = 0, int count = -1);
 // Bytes prepare() will hold for n conformations of L atoms.
 size_t prepared_bytes(int n, int L);
 
+// The device buffers of a prepared set as plain pointers (those of a
+// Prepared, or carved from any other memory of prepared_sizes' sizes).
+struct PreparedPtrs {
+    double* cent = nullptr;     // n x 3
+    double* rot = nullptr;      // n x 9
+    double* ref = nullptr;      // 3L + 6 (unused with a fixed reference)
+    double* partial = nullptr;  // reference-mean slices x 3L (unused with a fixed reference)
+    void* panels = nullptr;
+    double* consts = nullptr;   // kNumConsts x n_pad
+};
+struct PreparedSizes {
+    size_t cent, rot, ref, partial, panels, consts;
+};
+PreparedSizes prepared_sizes(int n, int L);
+
 // Reference, fit, pack (prep_device.hpp).  If `fixed_ref` (device, 3L + 6
 // doubles, the layout of Prepared::ref) is given, conformations are fitted
 // onto it, read in place (Prepared::ref stays unused); otherwise the reference
@@ -66,6 +81,9 @@ size_t prepared_bytes(int n, int L);
 // d_xyz is borrowed (Prepared::xyz).
 void prepare(Prepared& P, const double* d_xyz, int n, int L, GramMode mode,
              const double* fixed_ref, hipStream_t s);
+// The same into caller-provided buffers; returns the view the Gram run reads.
+PrepView prepare_into(const PreparedPtrs& p, const double* d_xyz, int n, int L, GramMode mode,
+                      const double* fixed_ref, hipStream_t s);
 
 // Flagging thresholds of the Gram epilogue's error model at atom count L.
 struct GramThresholds {
@@ -142,6 +160,18 @@ struct RunRecord {
 // Gram contraction + QCP epilogue + FP64 re-evaluation of flagged pairs over
 // rows of A x cols of B (or the upper triangle of A when triangle).  Asynchronous:
 // the flagged-pair count is copied to rec->h_count when rec is given.
+// Words of the flag bitmap of n_out outputs (a whole number of 16-byte units).
+inline uint64_t flag_words(uint64_t n_out) { return ((n_out + 31) / 32 + 3) / 4 * 4; }
+// Bytes of the flag region: the bitmap, a 16-byte count word, list_cap queued entries.
+inline size_t flag_region_bytes(uint64_t n_out, int list_cap)
+{
+    return (size_t)(flag_words(n_out) + 4) * 4 + (size_t)list_cap * 8;
+}
+// The Gram launch sequence with every buffer given: `order` (n_tiles tile
+// codes on the device), `flags` (flag_region_bytes(n_out, list_cap)); no
+// allocation, no host copy except the optional counts of `run` / `rec`.
+void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, const uint32_t* order,
+                   int n_tiles, uint32_t* flags, int list_cap, int max_wg, hipStream_t s, RunRecord* rec);
 void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScratch& sc,
               hipStream_t s, RunRecord* rec);
 

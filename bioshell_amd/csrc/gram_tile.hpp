@@ -57,6 +57,8 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
     const double Ld = (double)a.L;
     const double invL = 1.0 / Ld;
     const double tau2L = a.tau * a.tau * Ld;
+    // the solve's own error budget: a quarter of the flagging tolerance
+    const double sec_tol2L = 0.0625 * tau2L;
 
 #define BS_STASH(BASE)                                                   \
     _Pragma("unroll") for (int s_ = 0; s_ < 4; ++s_)                     \
@@ -72,19 +74,41 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
             // lane's block; slots sl and sl + 1 are solved together
             const int il = il0 + 8 * round + sl;
             const int i0 = tr * TILE + il, i1 = i0 + 1;
-            double E0[9], E1[9], c0[kNumConsts], c1[kNumConsts];
+            // the rows' constants are read from LDS where they are used
+            const double* c0 = sconst + il * 12;
+            const double* c1 = sconst + (il + 1) * 12;
+            // Newton on the secular form of the QCP polynomial from mu = 0 for
+            // both pairs in lockstep (straight-line code: the two chains
+            // interleave); a second step where the bracket is still wide, and
+            // the quartic where S = -B is not positive definite, run only in
+            // waves that need them
+            SecularBracket br[2];
+            {
+                double E0[9], E1[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                E0[q] = (double)sacc[(sl * 9 + q) * 64 + lane];
-                E1[q] = (double)sacc[((sl + 1) * 9 + q) * 64 + lane];
-            }
-#pragma unroll
-            for (int q = 0; q < kNumConsts; ++q) {
-                c0[q] = sconst[il * 12 + q];
-                c1[q] = sconst[(il + 1) * 12 + q];
+                for (int q = 0; q < 9; ++q) {
+                    E0[q] = (double)sacc[(sl * 9 + q) * 64 + lane];
+                    E1[q] = (double)sacc[((sl + 1) * 9 + q) * 64 + lane];
+                }
+                br[0] = secular_step(secular_system(E0, c0, cj), 0.0);
+                br[1] = secular_step(secular_system(E1, c1, cj), 0.0);
             }
             ShiftedOut o[2];
-            qcp_shifted2(E0, c0, E1, c1, cj, o[0], o[1]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                o[h] = secular_out(br[h], sec_tol2L);
+                if (!o[h].converged) {  // rebuilt from the accumulators in LDS
+                    double Eh[9];
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) Eh[q] = (double)sacc[((sl + h) * 9 + q) * 64 + lane];
+                    const double* ch = h ? c1 : c0;
+                    if (br[h].pd) {
+                        br[h] = secular_step(secular_system(Eh, ch, cj), br[h].lo);
+                        o[h] = secular_out(br[h], sec_tol2L);
+                    }
+                    if (!o[h].converged) o[h] = qcp_shifted(Eh, ch, cj);
+                }
+            }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int i = h ? i1 : i0;

@@ -395,77 +395,104 @@ BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj
 // |v|^2 ~ mu*/|S|, far below the tolerance for chains of a few hundred atoms;
 // short chains take a second step.  A pair whose S is not positive definite,
 // or whose bracket stays too wide, is solved with the quartic above.
-struct SecularPair {
+struct SecularSystem {
     double b00, b11, b22, b01, b02, b12;  // B
     double a0, a1, a2;                    // a
     double H;
-    double lo, hi;  // mu* in [lo, hi]
-    bool pd;        // S = -B positive definite
 };
 
-// One Newton step of f from mu = st.lo (st.lo = 0 at the start): returns
-// whether S + mu I was found positive definite (else the bracket is void).
-BS_HD bool secular_step(SecularPair& st)
+// B, a and H of one pair (as shifted_poly).
+BS_HD SecularSystem secular_system(const double* E, const double* ci, const double* cj)
 {
-    const double m = st.lo;
+    SecularSystem y;
+    const double trE = E[0] + E[4] + E[8];
+    y.b00 = ci[0] + cj[0] + 2.0 * (E[0] - trE);
+    y.b11 = ci[1] + cj[1] + 2.0 * (E[4] - trE);
+    y.b22 = ci[2] + cj[2] + 2.0 * (E[8] - trE);
+    y.b01 = ci[3] + cj[3] + (E[1] + E[3]);
+    y.b02 = ci[4] + cj[4] + (E[2] + E[6]);
+    y.b12 = ci[5] + cj[5] + (E[5] + E[7]);
+    y.a0 = (cj[6] - ci[6]) + (E[5] - E[7]);
+    y.a1 = (cj[7] - ci[7]) + (E[6] - E[2]);
+    y.a2 = (cj[8] - ci[8]) + (E[1] - E[3]);
+    y.H = (ci[9] + cj[9]) - trE;
+    return y;
+}
+
+// mu* in [lo, hi], and whether S was found positive definite on the way.
+struct SecularBracket {
+    double H, lo, hi;
+    bool pd;
+};
+
+// One Newton step of f from m (<= mu*): the new lower bound and g0(m) as the
+// upper bound; pd reports S + m I positive definite (else the bracket is void).
+BS_HD SecularBracket secular_step(const SecularSystem& y, double m)
+{
     // S + m I = -(B - m I); adj(-X) = adj(X) for 3x3, det(-X) = -det(X)
-    const double c00 = st.b00 - m, c11 = st.b11 - m, c22 = st.b22 - m;
-    const double A00 = c11 * c22 - st.b12 * st.b12;
-    const double A11 = c00 * c22 - st.b02 * st.b02;
-    const double A22 = c00 * c11 - st.b01 * st.b01;
-    const double A01 = st.b02 * st.b12 - st.b01 * c22;
-    const double A02 = st.b01 * st.b12 - st.b02 * c11;
-    const double A12 = st.b01 * st.b02 - c00 * st.b12;
-    const double det = -(c00 * A00 + st.b01 * A01 + st.b02 * A02);  // det(S + m I)
-    const double w0 = A00 * st.a0 + A01 * st.a1 + A02 * st.a2;
-    const double w1 = A01 * st.a0 + A11 * st.a1 + A12 * st.a2;
-    const double w2 = A02 * st.a0 + A12 * st.a1 + A22 * st.a2;
-    const double aw = st.a0 * w0 + st.a1 * w1 + st.a2 * w2;  // det * g0(m)
-    const double ww = w0 * w0 + w1 * w1 + w2 * w2;           // det^2 * |v|^2
+    const double c00 = y.b00 - m, c11 = y.b11 - m, c22 = y.b22 - m;
+    const double A00 = c11 * c22 - y.b12 * y.b12;
+    const double A11 = c00 * c22 - y.b02 * y.b02;
+    const double A22 = c00 * c11 - y.b01 * y.b01;
+    const double A01 = y.b02 * y.b12 - y.b01 * c22;
+    const double A02 = y.b01 * y.b12 - y.b02 * c11;
+    const double A12 = y.b01 * y.b02 - c00 * y.b12;
+    const double det = -(c00 * A00 + y.b01 * A01 + y.b02 * A02);  // det(S + m I)
+    const double w0 = A00 * y.a0 + A01 * y.a1 + A02 * y.a2;
+    const double w1 = A01 * y.a0 + A11 * y.a1 + A12 * y.a2;
+    const double w2 = A02 * y.a0 + A12 * y.a1 + A22 * y.a2;
+    const double aw = y.a0 * w0 + y.a1 * w1 + y.a2 * w2;  // det * g0(m)
+    const double ww = w0 * w0 + w1 * w1 + w2 * w2;        // det^2 * |v|^2
+    SecularBracket b;
+    b.H = y.H;
     // Sylvester: leading minors of S + m I are -c00, A22, det
-    const bool pd = (c00 < 0.0) & (A22 > 0.0) & (det > 0.0);
+    b.pd = (c00 < 0.0) & (A22 > 0.0) & (det > 0.0);
     // new = m - (m - g0) / (1 + |v|^2) = (m ww + aw det) / (det^2 + ww), and
     // g0(m) = aw / det: both quotients from one reciprocal of det (det^2 + ww)
     const double den = det * det + ww;
     const double r = newton_div(1.0, det * den);
-    st.lo = (m * ww + aw * det) * (det * r);
-    st.hi = aw * (den * r);
-    return pd;
-}
-
-// B, a and H of one pair (as shifted_poly), then the first step from 0.
-BS_HD SecularPair secular_first(const double* E, const double* ci, const double* cj)
-{
-    SecularPair st;
-    const double trE = E[0] + E[4] + E[8];
-    st.b00 = ci[0] + cj[0] + 2.0 * (E[0] - trE);
-    st.b11 = ci[1] + cj[1] + 2.0 * (E[4] - trE);
-    st.b22 = ci[2] + cj[2] + 2.0 * (E[8] - trE);
-    st.b01 = ci[3] + cj[3] + (E[1] + E[3]);
-    st.b02 = ci[4] + cj[4] + (E[2] + E[6]);
-    st.b12 = ci[5] + cj[5] + (E[5] + E[7]);
-    st.a0 = (cj[6] - ci[6]) + (E[5] - E[7]);
-    st.a1 = (cj[7] - ci[7]) + (E[6] - E[2]);
-    st.a2 = (cj[8] - ci[8]) + (E[1] - E[3]);
-    st.H = (ci[9] + cj[9]) - trE;
-    st.lo = 0.0;
-    st.pd = secular_step(st);
-    return st;
+    b.lo = (m * ww + aw * det) * (det * r);
+    b.hi = aw * (den * r);
+    return b;
 }
 
 // The result of a bracket: mu* is taken at its midpoint; converged when S is
 // positive definite and the half-width h, an error bound of H - mu, moves the
 // RMSD by at most tol: h <= tol L RMSD, i.e. h^2 <= tol2L * (L RMSD^2) with
 // tol2L = tol^2 L and L RMSD^2 = 2 (H - mu).  A NaN or negative h fails.
-BS_HD ShiftedOut secular_out(const SecularPair& st, double tol2L)
+BS_HD ShiftedOut secular_out(const SecularBracket& b, double tol2L)
 {
     ShiftedOut o;
-    const double mu = 0.5 * (st.lo + st.hi), half = 0.5 * (st.hi - st.lo);
-    const double d = st.H - mu;
+    const double mu = 0.5 * (b.lo + b.hi), half = 0.5 * (b.hi - b.lo);
+    const double d = b.H - mu;
     o.rmsd2_L = 2.0 * (d > 0.0 ? d : 0.0);
-    o.H = st.H;
+    o.H = b.H;
     o.iterations = 1;
-    o.converged = st.pd & (half >= 0.0) & (half * half <= tol2L * o.rmsd2_L);
+    o.converged = b.pd & (half >= 0.0) & (half * half <= tol2L * o.rmsd2_L);
+    return o;
+}
+
+// The sequence the Gram epilogue runs for a pair: one secular step from 0; a
+// second from the new lower bound while S is positive definite but the
+// bracket too wide; the quartic (qcp_shifted) when that still fails.
+// `steps` receives 1 or 2, or 0 when the quartic solved it.
+BS_HD ShiftedOut qcp_secular(const double* E, const double* ci, const double* cj, double tol2L,
+                             int* steps = nullptr)
+{
+    const SecularSystem y = secular_system(E, ci, cj);
+    SecularBracket b = secular_step(y, 0.0);
+    ShiftedOut o = secular_out(b, tol2L);
+    int n = 1;
+    if (!o.converged && b.pd) {
+        b = secular_step(y, b.lo);
+        o = secular_out(b, tol2L);
+        n = 2;
+    }
+    if (!o.converged) {
+        o = qcp_shifted(E, ci, cj);
+        n = 0;
+    }
+    if (steps) *steps = n;
     return o;
 }
 

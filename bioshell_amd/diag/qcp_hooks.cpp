@@ -35,23 +35,13 @@ float bsi_qcp_rmsd_f32(const float* M, float Ga, float Gb, int n)
 
 extern "C" {
 
-// The secular-form solve with its fallbacks (as the Gram epilogue runs it):
-// out[0] = 2 (H - mu), out[1] = H, out[2] = steps taken (1, 2; 0 when the
-// quartic solved it), out[3] = converged (1/0).
+// The secular-form solve with its fallbacks (the Gram epilogue's sequence):
+// out[0] = 2 (H - mu), out[1] = H, out[2] = secular steps taken (1, 2; 0 when
+// the quartic solved it), out[3] = converged (1/0).
 void bsi_qcp_secular(const double* E, const double* ci, const double* cj, double tol2L, double* out)
 {
-    bs::SecularPair st = bs::secular_first(E, ci, cj);
-    bs::ShiftedOut o = bs::secular_out(st, tol2L);
-    double steps = 1.0;
-    if (!o.converged && st.pd) {
-        st.pd = bs::secular_step(st);
-        o = bs::secular_out(st, tol2L);
-        steps = 2.0;
-    }
-    if (!o.converged) {
-        o = bs::qcp_shifted(E, ci, cj);
-        steps = 0.0;
-    }
+    int steps = 0;
+    const bs::ShiftedOut o = bs::qcp_secular(E, ci, cj, tol2L, &steps);
     out[0] = o.rmsd2_L;
     out[1] = o.H;
     out[2] = steps;
