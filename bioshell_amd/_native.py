@@ -84,6 +84,8 @@ SIGNATURES = {
     "bs_superpose_f64": (c_status, [c_vp, c_dp, c_dp, c_size, c_dp, c_dp, c_dp]),
     "bs_crmsd_all_vs_all": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_vp, ctypes.c_int]),
     "bs_crmsd_all_vs_all_device": (c_status, [c_vp, c_vp, c_size, c_size, ctypes.c_int, c_vp, ctypes.c_int]),
+    "bs_crmsd_all_vs_all_device_noalloc": (c_status, [c_vp, c_vp, c_size, c_size, ctypes.c_int, c_vp,
+                                                      ctypes.c_int]),
     "bs_crmsd_all_vs_all_streamed": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_size, c_fp]),
     "bs_crmsd_batch": (c_status, [c_vp, ctypes.POINTER(c_dp), ctypes.POINTER(c_size), ctypes.POINTER(c_size),
                                   c_size, ctypes.c_int, ctypes.POINTER(c_fp)]),

@@ -75,6 +75,7 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             fail(BS_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950", device,
                  prop.gcnArchName);
+        noalloc_init(device);  // the no-allocation entry point's scratch address on this device
         auto ctx = std::make_unique<bs_context>();
         ctx->device = device;
         ctx->n_cu = prop.multiProcessorCount;

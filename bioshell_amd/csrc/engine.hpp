@@ -16,6 +16,10 @@ namespace bs {
 // HBM budget, created on first use and kept until the process ends).
 bs_context* resolve_context(bs_context* ctx);
 
+// Takes the current device's address of the no-allocation entry point's
+// scratch array (once per device; called when a context is created on it).
+void noalloc_init(int device);
+
 // Conformations of one atom count, prepared on the device for the Gram path:
 // the raw coordinates they came from (borrowed: they must outlive every run
 // that reads this set, for the FP64 re-evaluation), centroids, rotations

@@ -178,6 +178,22 @@ bs_status bs_crmsd_all_vs_all_device(bs_context* ctx, const double* d_xyz, size_
                                      size_t n_atoms, bs_method method, void* d_out_upper,
                                      int out_f64);
 
+/* The same as bs_crmsd_all_vs_all_device, without allocating anything.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its whole workspace -- prepared operand panels and
+ * constants, centroids, rotations, the reference, the tile order, the flag
+ * bitmap and queue -- lives in one fixed 64 MiB array of the library's device
+ * memory, which every GPU the library runs on holds until the process ends;
+ * inputs that need more return BS_ERR_OUT_OF_BUDGET with the size (10,000
+ * conformations of 300 atoms need about 47 MB).  Results equal
+ * bs_crmsd_all_vs_all_device's bit for bit.  Every call on one GPU shares that
+ * array: calls that overlap on a GPU race and must be ordered by the caller.
+ * No run statistics are recorded.  Asynchronous on the context's stream. */
+bs_status bs_crmsd_all_vs_all_device_noalloc(bs_context* ctx, const double* d_xyz, size_t n_conf,
+                                             size_t n_atoms, bs_method method, void* d_out_upper,
+                                             int out_f64);
+
 /* Streams the pair space in tile x tile blocks (e.g. 4096) and copies each
  * finished block of the packed triangle to the host while the next one is
  * computed; out_upper is a host f32 buffer of n(n-1)/2 values. */
