@@ -5,21 +5,28 @@
 
 namespace bs {
 
-// LDS bytes the epilogue of a TILE x TILE tile needs (see below).
+// LDS bytes the epilogue of a TILE x TILE tile needs: the constants of the
+// tile's rows and columns, 12 doubles each.
 template <int TILE>
 constexpr int epilogue_lds_bytes()
 {
-    return 2 * TILE * 12 * 8 + (TILE / 32) * (TILE / 32) * 4 * 9 * 64 * 4;
+    return 2 * TILE * 12 * 8;
 }
 
 // The FP64 QCP epilogue of a TILE x TILE tile whose accumulators the calling
 // waves hold: TILE 64 is four waves of a work-group (wave w owns rows
 // 32 (w/2) .. +31 and columns 32 (w%2) .. +31), TILE 32 is one wave on its own.
-// LDS layout (bytes) at `lds`, which the caller no longer needs:
-//   [0, 2 TILE 96)        constants of the tile's rows and columns, 12 doubles each
-//   then, per wave        4 accumulator slots x 9 values x 64 lanes (f32)
-// Accumulators go through LDS in 4 rounds of 4 slots so that one copy of the
-// FP64 solve serves all 16 pairs a lane owns.
+// The rows' and columns' constants go to LDS at `lds` (which the caller no
+// longer needs); each lane then solves its 16 pairs straight from its
+// accumulator registers, all 16 unrolled so that their independent FP64
+// chains interleave.  Accumulator slot r of a lane holds row
+// 8 (r/4) + 4 (lane/32) + r%4 of the wave's block and column lane % 32.
+//
+// The solve is Newton on the secular form of the QCP polynomial from mu = 0
+// (qcp.hpp), with a second step in the waves where a bracket is still wide;
+// a pair whose S = -B is not positive definite or whose bracket stays wide
+// is flagged, like every pair the error model cannot vouch for, and is
+// re-evaluated in FP64 from the coordinates.
 template <int TILE>
 __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x16 (&acc)[9],
                                          char* lds, int thread)
@@ -30,9 +37,7 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
     const int tid = kWaves == 1 ? lane : thread;
     const int wv = kWaves == 1 ? 0 : (thread >> 6);
     const int wr = wv >> 1, wc = wv & 1;
-    constexpr int kConstBytes = 2 * TILE * 12 * 8;
     double* sconst = reinterpret_cast<double*>(lds);
-    float* sacc = reinterpret_cast<float*>(lds + kConstBytes) + wv * (4 * 9 * 64);
     if constexpr (kWaves == 1) wave_lds_sync();  // earlier reads of this region are done
     for (int v = tid; v < kNumConsts * 2 * TILE; v += kThreads) {
         const int q = v / (2 * TILE), r = v % (2 * TILE);
@@ -59,82 +64,38 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
     const double tau2L = a.tau * a.tau * Ld;
     // the solve's own error budget: a quarter of the flagging tolerance
     const double sec_tol2L = 0.0625 * tau2L;
-
-#define BS_STASH(BASE)                                                   \
-    _Pragma("unroll") for (int s_ = 0; s_ < 4; ++s_)                     \
-        _Pragma("unroll") for (int q_ = 0; q_ < 9; ++q_)                 \
-            sacc[(s_ * 9 + q_) * 64 + lane] = acc[q_][(BASE) + s_];
-
 #pragma unroll
-    for (int round = 0; round < 4; ++round) {
-        BS_STASH(4 * round)  // each lane reads back only its own slots: no sync
-#pragma unroll 1
-        for (int sl = 0; sl < 4; sl += 2) {
-            // accumulator slot r = 4 round + sl holds row 8 (r/4) + r%4 of the
-            // lane's block; slots sl and sl + 1 are solved together
-            const int il = il0 + 8 * round + sl;
-            const int i0 = tr * TILE + il, i1 = i0 + 1;
-            // the rows' constants are read from LDS where they are used
-            const double* c0 = sconst + il * 12;
-            const double* c1 = sconst + (il + 1) * 12;
-            // Newton on the secular form of the QCP polynomial from mu = 0 for
-            // both pairs in lockstep (straight-line code: the two chains
-            // interleave); a second step where the bracket is still wide, and
-            // the quartic where S = -B is not positive definite, run only in
-            // waves that need them
-            SecularBracket br[2];
-            {
-                double E0[9], E1[9];
+    for (int r = 0; r < 16; ++r) {
+        const int il = il0 + 8 * (r / 4) + (r % 4);
+        const int i = tr * TILE + il;
+        const double* ci = sconst + il * 12;  // read from LDS where used (broadcast)
+        double E[9];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    E0[q] = (double)sacc[(sl * 9 + q) * 64 + lane];
-                    E1[q] = (double)sacc[((sl + 1) * 9 + q) * 64 + lane];
-                }
-                br[0] = secular_step(secular_system(E0, c0, cj), 0.0);
-                br[1] = secular_step(secular_system(E1, c1, cj), 0.0);
-            }
-            ShiftedOut o[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                o[h] = secular_out(br[h], sec_tol2L);
-                if (!o[h].converged) {  // rebuilt from the accumulators in LDS
-                    double Eh[9];
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) Eh[q] = (double)sacc[((sl + h) * 9 + q) * 64 + lane];
-                    const double* ch = h ? c1 : c0;
-                    if (br[h].pd) {
-                        br[h] = secular_step(secular_system(Eh, ch, cj), br[h].lo);
-                        o[h] = secular_out(br[h], sec_tol2L);
-                    }
-                    if (!o[h].converged) o[h] = qcp_shifted(Eh, ch, cj);
-                }
-            }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int i = h ? i1 : i0;
-                const double* ci = h ? c1 : c0;
-                const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
-                if (ok) {
-                    const double sgg = ci[10] * cj[10];
-                    const double err = a.acc_coef * sgg;
-                    // flag: unconverged / near-double root, near-duplicate
-                    // residuals, estimated error above tau (compared squared:
-                    // no FP64 sqrt), or residuals too large for the split
-                    const bool flag = !o[h].converged || (o[h].H < a.kappa_dup * a.u_rep * sgg) ||
-                                      (err * err > tau2L * o[h].rmsd2_L) ||
-                                      (2.0 * (ci[9] + cj[9]) > a.rep_g_max);
-                    const int64_t oi =
-                        a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
-                    if (a.out_f64)
-                        static_cast<double*>(a.out)[oi] = sqrt(o[h].rmsd2_L * invL);
-                    else
-                        static_cast<float*>(a.out)[oi] = sqrtf((float)(o[h].rmsd2_L * invL));
-                    flag_output(a, flag, oi);
-                }
-            }
+        for (int q = 0; q < 9; ++q) E[q] = (double)acc[q][r];
+        const SecularSystem y = secular_system(E, ci, cj);
+        SecularBracket br = secular_step(y, 0.0);
+        PairSolve o = secular_out(br, sec_tol2L);
+        if (!o.converged & br.pd) {  // rare for a few hundred atoms; taken by waves that need it
+            br = secular_step(y, br.lo);
+            o = secular_out(br, sec_tol2L);
+        }
+        const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
+        if (ok) {
+            const double sgg = ci[10] * cj[10];
+            const double err = a.acc_coef * sgg;
+            // flag: no verified root, near-duplicate residuals, estimated
+            // error above tau (compared squared: no FP64 sqrt), or residuals
+            // too large for the split
+            const bool flag = !o.converged || (o.H < a.kappa_dup * a.u_rep * sgg) ||
+                              (err * err > tau2L * o.rmsd2_L) || (2.0 * (ci[9] + cj[9]) > a.rep_g_max);
+            const int64_t oi = a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
+            if (a.out_f64)
+                static_cast<double*>(a.out)[oi] = sqrt(o.rmsd2_L * invL);
+            else
+                static_cast<float*>(a.out)[oi] = sqrtf((float)(o.rmsd2_L * invL));
+            flag_output(a, flag, oi);
         }
     }
-#undef BS_STASH
 }
 
 // s_waitcnt with the vector-memory counter limited to N (gfx9 encoding:
@@ -148,18 +109,89 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
     __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(SRC),     \
                                      (void __attribute__((address_space(3)))*)(DST), 16, 0, 0)
 
+// Operand fragments of one atom block for one wave, in registers: the six
+// 16-byte words of its row block (A) and of its column block (B).  BF16x3:
+// word 2c / 2c+1 are the hi / lo planes of coordinate c; FP32: the two words
+// of coordinate c's plane.
+struct OperandRegs {
+    uint4 a[6], b[6];
+};
+
+template <int MODE>
+__device__ __forceinline__ void load_operands(const uint4* __restrict__ A, const uint4* __restrict__ B,
+                                              int lane, OperandRegs& r)
+{
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+        const int w = MODE == kGramBF16x3 ? u * 64 + lane : (u >> 1) * 128 + 2 * lane + (u & 1);
+        r.a[u] = A[w];
+        r.b[u] = B[w];
+    }
+}
+
+// The MFMAs of one atom block on operands in registers (the same products,
+// in the same order, as mfma_block).
+template <int MODE>
+__device__ __forceinline__ void mfma_regs(const OperandRegs& r, f32x16 (&acc)[9])
+{
+    if constexpr (MODE == kGramBF16x3) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    as_bf16x8(r.a[2 * c]), as_bf16x8(r.b[2 * d]), acc[3 * c + d], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    as_bf16x8(r.a[2 * c]), as_bf16x8(r.b[2 * d + 1]), acc[3 * c + d], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    as_bf16x8(r.a[2 * c + 1]), as_bf16x8(r.b[2 * d]), acc[3 * c + d], 0, 0, 0);
+    } else {
+        float fa[3][8], fb[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint4 a0 = r.a[2 * c], a1 = r.a[2 * c + 1], b0 = r.b[2 * c], b1 = r.b[2 * c + 1];
+            fa[c][0] = __uint_as_float(a0.x); fa[c][1] = __uint_as_float(a0.y);
+            fa[c][2] = __uint_as_float(a0.z); fa[c][3] = __uint_as_float(a0.w);
+            fa[c][4] = __uint_as_float(a1.x); fa[c][5] = __uint_as_float(a1.y);
+            fa[c][6] = __uint_as_float(a1.z); fa[c][7] = __uint_as_float(a1.w);
+            fb[c][0] = __uint_as_float(b0.x); fb[c][1] = __uint_as_float(b0.y);
+            fb[c][2] = __uint_as_float(b0.z); fb[c][3] = __uint_as_float(b0.w);
+            fb[c][4] = __uint_as_float(b1.x); fb[c][5] = __uint_as_float(b1.y);
+            fb[c][6] = __uint_as_float(b1.z); fb[c][7] = __uint_as_float(b1.w);
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    acc[3 * c + d] =
+                        __builtin_amdgcn_mfma_f32_32x32x2f32(fa[c][s], fb[d][s], acc[3 * c + d], 0, 0, 0);
+    }
+}
+
 // One 64 x 64 tile of pairs by a 4-wave work-group.  K loop over atom
 // blocks with a three-stage LDS ring fed by direct global->LDS loads (no
-// registers held for staging): stage k holds the four operand blocks (A0, A1,
-// B0, B1: 6 KiB each) of atom block k; wave w streams block w.  Step k waits
-// for its own loads of stage k (vmcnt: the 6 loads of stage k+1 may still be
-// in flight) and for its own LDS reads of stage k-1 (lgkmcnt 0), meets the
-// others at a barrier (after which stage k-1 is free), issues atom block k+2
-// into that stage, and runs the 27 (or 72) MFMAs of block k.  Steps are unrolled by three so that every LDS access names its
-// stage statically; the compiler can then see that reads of stage k do not
-// depend on the loads still in flight into the other stages.  Loads past the
-// last block re-read the last block (in bounds, unused); the loop drains
-// them before the epilogue reuses the LDS.
+// registers held for staging): a stage holds the four operand blocks (A0,
+// A1, B0, B1: 6 KiB each) of one atom block; wave w streams block w.  The
+// operands of block k are read from LDS into registers during step k-1, so
+// the MFMAs of a step never wait on LDS.  Step k waits for its own loads of
+// block k+1 (vmcnt: the 6 loads of block k+2 may still be in flight) and for
+// its own LDS reads of block k (lgkmcnt 0), meets the others at a barrier
+// (after which every wave has block k in registers and block k+1 in LDS),
+// issues block k+3 into the stage block k came from, reads block k+1 into the
+// other register set, and runs the 27 (or 72) MFMAs of block k.  Steps are
+// unrolled by six so that every LDS stage and register set is named
+// statically.  Loads past the last block re-read the last block (in bounds,
+// unused); the loop drains them before the epilogue reuses the LDS.
 template <int MODE>
 __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
                                             uint4 (&lds)[3][4][kPanelBlockU4], int thread)
@@ -179,14 +211,15 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
         _Pragma("unroll") for (int u_ = 0; u_ < 6; ++u_)                          \
             BS_GLOAD_LDS(g_ + u_ * 64, &lds[ST][wv][u_ * 64]);                    \
     }
-#define BS_STEP(KB, SC, SN)                                                       \
+#define BS_STEP(KB, SCUR, SNEXT, RCUR, RNEXT)                                     \
     {                                                                             \
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                    \
         BS_WAIT_VMCNT_LDS0(6);                                                    \
         __builtin_amdgcn_s_barrier();                                             \
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                    \
-        BS_ISSUE((KB) + 2, SN)                                                    \
-        if ((KB) < n_kblk) mfma_block<MODE>(lds[SC][wr], lds[SC][2 + wc], lane, acc); \
+        BS_ISSUE((KB) + 3, SCUR)                                                  \
+        load_operands<MODE>(lds[SNEXT][wr], lds[SNEXT][2 + wc], lane, RNEXT);    \
+        mfma_regs<MODE>(RCUR, acc);                                               \
     }
 
     f32x16 acc[9];
@@ -197,15 +230,28 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
 
     // the K loop runs at a raised wave priority and the FP64 epilogue at the
     // normal one: with two work-groups per CU in different phases, the
-    // sequencer then favours the MFMA stream (measured 2 % faster than equal
-    // priorities, 0.6 % for the opposite choice)
+    // sequencer then favours the MFMA stream
     __builtin_amdgcn_s_setprio(2);
+    OperandRegs ra, rb;
     BS_ISSUE(0, 0)
     BS_ISSUE(1, 1)
-    for (int kb = 0; kb < n_kblk; kb += 3) {
-        BS_STEP(kb, 0, 2)
-        BS_STEP(kb + 1, 1, 0)
-        BS_STEP(kb + 2, 2, 1)
+    BS_ISSUE(2, 2)
+    BS_WAIT_VMCNT(12);  // own loads of block 0 landed
+    __syncthreads();
+    load_operands<MODE>(lds[0][wr], lds[0][2 + wc], lane, ra);
+    for (int kb = 0;; kb += 6) {
+        BS_STEP(kb, 0, 1, ra, rb)
+        if (kb + 1 >= n_kblk) break;
+        BS_STEP(kb + 1, 1, 2, rb, ra)
+        if (kb + 2 >= n_kblk) break;
+        BS_STEP(kb + 2, 2, 0, ra, rb)
+        if (kb + 3 >= n_kblk) break;
+        BS_STEP(kb + 3, 0, 1, rb, ra)
+        if (kb + 4 >= n_kblk) break;
+        BS_STEP(kb + 4, 1, 2, ra, rb)
+        if (kb + 5 >= n_kblk) break;
+        BS_STEP(kb + 5, 2, 0, rb, ra)
+        if (kb + 6 >= n_kblk) break;
     }
     BS_WAIT_VMCNT(0);  // no load may still write the LDS the epilogue reuses
     __syncthreads();

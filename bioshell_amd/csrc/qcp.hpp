@@ -10,7 +10,7 @@
 //                  Used by the per-pair kernels (T = double or float) and by
 //                  the FP64 re-evaluation of flagged pairs.
 //
-//  qcp_shifted()   the same root, solved in a frame where no large numbers
+//  qcp_secular()   the same root, solved in a frame where no large numbers
 //                  cancel.  Every conformation x_i of an ensemble is first
 //                  superposed onto a common reference r and written
 //                  x_i = r + e_i.  Then, exactly,
@@ -26,6 +26,7 @@
 //                  in [0, H], and E0 - lambda = H - mu.  Only E_ij (residual
 //                  magnitudes) comes from the MFMA Gram contraction; H - mu is
 //                  formed without subtracting two numbers of size |x|^2.
+//                  Used by the Gram epilogue.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -207,173 +208,12 @@ BS_HD T qcp_rmsd(const T* M, T Ga, T Gb, int n, int max_iter = 50)
 //   10    sqrt(g_i)
 constexpr int kNumConsts = 11;
 
-struct ShiftedOut {
-    double rmsd2_L;  // 2 (H - mu), i.e. L * RMSD^2
+// A pair's solve: 2 (H - mu) = L RMSD^2, H, and whether the root is verified.
+struct PairSolve {
+    double rmsd2_L;
     double H;
-    int iterations;
     bool converged;
 };
-
-// Coefficients of det(mu I - K') = mu^4 + p3 mu^3 + p2 mu^2 + p1 mu + p0 for
-// one pair, from its residual Gram E (row-major, E[3p+q] = sum_k e_i,kp e_j,kq)
-// and both conformations' constants.
-struct ShiftedPoly {
-    double p3, p2, p1, p0, H, aa;
-};
-
-BS_HD ShiftedPoly shifted_poly(const double* E, const double* ci, const double* cj)
-{
-    const double trE = E[0] + E[4] + E[8];
-    // B = Bh_i + Bh_j + (E + E^T - 2 tr E I)
-    const double b00 = ci[0] + cj[0] + 2.0 * (E[0] - trE);
-    const double b11 = ci[1] + cj[1] + 2.0 * (E[4] - trE);
-    const double b22 = ci[2] + cj[2] + 2.0 * (E[8] - trE);
-    const double b01 = ci[3] + cj[3] + (E[1] + E[3]);
-    const double b02 = ci[4] + cj[4] + (E[2] + E[6]);
-    const double b12 = ci[5] + cj[5] + (E[5] + E[7]);
-    // a = w(C_j) - w(C_i) + w(E)
-    const double a0 = (cj[6] - ci[6]) + (E[5] - E[7]);
-    const double a1 = (cj[7] - ci[7]) + (E[6] - E[2]);
-    const double a2 = (cj[8] - ci[8]) + (E[1] - E[3]);
-    // invariants of B and its adjugate
-    const double A00 = b11 * b22 - b12 * b12;
-    const double A11 = b00 * b22 - b02 * b02;
-    const double A22 = b00 * b11 - b01 * b01;
-    const double A01 = b02 * b12 - b01 * b22;
-    const double A02 = b01 * b12 - b02 * b11;
-    const double A12 = b01 * b02 - b00 * b12;
-    const double t1 = b00 + b11 + b22;
-    const double t2 = A00 + A11 + A22;
-    const double t3 = b00 * A00 + b01 * A01 + b02 * A02;
-    const double aa = a0 * a0 + a1 * a1 + a2 * a2;
-    const double aBa = b00 * a0 * a0 + b11 * a1 * a1 + b22 * a2 * a2 +
-                       2.0 * (b01 * a0 * a1 + b02 * a0 * a2 + b12 * a1 * a2);
-    const double aAa = A00 * a0 * a0 + A11 * a1 * a1 + A22 * a2 * a2 +
-                       2.0 * (A01 * a0 * a1 + A02 * a0 * a2 + A12 * a1 * a2);
-    ShiftedPoly s;
-    s.p3 = -t1;
-    s.p2 = t2 - aa;
-    s.p1 = t1 * aa - aBa - t3;
-    s.p0 = -aAa;
-    s.H = (ci[9] + cj[9]) - trE;
-    s.aa = aa;
-    return s;
-}
-
-BS_HD double poly_at(const ShiftedPoly& s, double m)
-{
-    return (((m + s.p3) * m + s.p2) * m + s.p1) * m + s.p0;
-}
-BS_HD double dpoly_at(const ShiftedPoly& s, double m)
-{
-    return ((4.0 * m + 3.0 * s.p3) * m + 2.0 * s.p2) * m + s.p1;
-}
-
-// Newton state of one pair.
-struct ShiftedIter {
-    double mu, dp, step;
-    int it;
-    bool conv, stuck, below, restarted;
-};
-
-// Start: near mu = 0 the quartic is close to p1 mu + p0, so -p0/p1 is the
-// root to first order; if that is not in [0, H) the start is the upper bound
-// H (lambda <= E0).  Whether the iterate reached is the largest root is
-// decided at the end (shifted_verify).
-BS_HD double shifted_start(const ShiftedPoly& s)
-{
-    const double Hp = s.H > 0.0 ? s.H : 0.0;
-    const double m = -newton_div(s.p0, s.p1);
-    const bool use = (s.p1 > 0.0) & (m >= 0.0) & (m < Hp);
-    return use ? m : Hp;
-}
-
-// One Newton step from st.mu (branch-free: a non-positive slope stops the
-// iteration and marks the pair stuck).
-BS_HD void shifted_newton(const ShiftedPoly& s, ShiftedIter& st)
-{
-    const double p = poly_at(s, st.mu);
-    const double dp = dpoly_at(s, st.mu);
-    st.stuck = st.stuck | !(dp > 0.0);
-    st.step = dp > 0.0 ? newton_div(p, dp) : 0.0;
-    st.mu -= st.step;
-    ++st.it;
-}
-
-// At the iterate mu reached by the step st.step: converged when the slope and
-// the Taylor coefficients P''/2 and P'''/6 there are positive (then no root of
-// the real-rooted quartic lies above mu, so mu is the largest root) and the
-// error that Newton's quadratic convergence predicts, |P''/(2P')| step^2, is
-// below the tolerance.  Leaves P'(mu) in st.dp.
-BS_HD void shifted_verify(const ShiftedPoly& s, ShiftedIter& st)
-{
-    const double m = st.mu;
-    const double d1 = dpoly_at(s, m);
-    const double t2 = (6.0 * m + 3.0 * s.p3) * m + s.p2;
-    const double t3 = 4.0 * m + s.p3;
-    const double Hp = s.H > 0.0 ? s.H : 0.0;
-    const double tol = 1e-13 * fabs(m) + 1e-15 * Hp;
-    // |t2 / d1| step^2 <= tol, without a division
-    const bool small = fabs(t2) * (st.step * st.step) <= tol * fabs(d1);
-    st.dp = d1;
-    st.below = !((d1 > 0.0) & (t2 >= 0.0) & (t3 >= 0.0));
-    st.conv = !st.stuck & !st.below & small;
-}
-
-// Iterates after an unverified first step (rare): a root that is not the
-// largest (a negative Taylor coefficient) or a non-positive slope restarts
-// the iteration once from H, which bounds every root from above, so Newton
-// then decreases monotonically to the largest root.
-BS_HD void shifted_continue(const ShiftedPoly& s, ShiftedIter& st, int max_iter)
-{
-    while (!st.conv && st.it < max_iter) {
-        if ((st.below || st.stuck) && !st.restarted) {
-            st.mu = s.H > 0.0 ? s.H : 0.0;
-            st.restarted = true;
-            st.stuck = false;
-        } else if (st.stuck) {
-            break;
-        }
-        shifted_newton(s, st);
-        shifted_verify(s, st);
-    }
-}
-
-BS_HD ShiftedOut shifted_finish(const ShiftedPoly& s, const ShiftedIter& st)
-{
-    bool conv = st.conv;
-    if (s.aa == 0.0 && !(st.mu > 0.0)) conv = true;  // exactly aligned pair: mu = max(0, lmax(B))
-    // near-double root (a small P' at the root): re-evaluate in FP64
-    const double m = st.mu;
-    const double scale =
-        fabs(4.0 * m * m * m) + fabs(3.0 * s.p3 * m * m) + fabs(2.0 * s.p2 * m) + fabs(s.p1);
-    if (fabs(st.dp) <= 1e-4 * scale) conv = false;
-    ShiftedOut o;
-    const double d = s.H - m;
-    o.rmsd2_L = 2.0 * (d > 0.0 ? d : 0.0);
-    o.H = s.H;
-    o.iterations = st.it;
-    o.converged = conv;
-    return o;
-}
-
-// Solves for the largest root of det(mu I - K') for one pair: one Newton step
-// from the start, then more only while the verification fails.
-BS_HD void shifted_solve(const ShiftedPoly& s, ShiftedIter& st, int max_iter)
-{
-    shifted_newton(s, st);
-    shifted_verify(s, st);
-    shifted_continue(s, st, max_iter);
-}
-
-BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj,
-                             int max_iter = 40)
-{
-    const ShiftedPoly s = shifted_poly(E, ci, cj);
-    ShiftedIter st{shifted_start(s), 0.0, 0.0, 0, false, false, false, false};
-    shifted_solve(s, st, max_iter);
-    return shifted_finish(s, st);
-}
 
 // ---------------------------------------------------------------------------
 // The same largest root through the Schur complement of K' = [[0, a^T], [a, B]]:
@@ -394,14 +234,16 @@ BS_HD ShiftedOut qcp_shifted(const double* E, const double* ci, const double* cj
 // conformations) and one step leaves a bracket of relative width
 // |v|^2 ~ mu*/|S|, far below the tolerance for chains of a few hundred atoms;
 // short chains take a second step.  A pair whose S is not positive definite,
-// or whose bracket stays too wide, is solved with the quartic above.
+// or whose bracket stays too wide, is left unconverged: the Gram path then
+// re-evaluates it with the textbook form above from the coordinates.
 struct SecularSystem {
     double b00, b11, b22, b01, b02, b12;  // B
     double a0, a1, a2;                    // a
     double H;
 };
 
-// B, a and H of one pair (as shifted_poly).
+// B, a and H of one pair from its residual Gram E (row-major,
+// E[3p+q] = sum_k e_i,kp e_j,kq) and both conformations' constants.
 BS_HD SecularSystem secular_system(const double* E, const double* ci, const double* cj)
 {
     SecularSystem y;
@@ -460,61 +302,35 @@ BS_HD SecularBracket secular_step(const SecularSystem& y, double m)
 // positive definite and the half-width h, an error bound of H - mu, moves the
 // RMSD by at most tol: h <= tol L RMSD, i.e. h^2 <= tol2L * (L RMSD^2) with
 // tol2L = tol^2 L and L RMSD^2 = 2 (H - mu).  A NaN or negative h fails.
-BS_HD ShiftedOut secular_out(const SecularBracket& b, double tol2L)
+BS_HD PairSolve secular_out(const SecularBracket& b, double tol2L)
 {
-    ShiftedOut o;
+    PairSolve o;
     const double mu = 0.5 * (b.lo + b.hi), half = 0.5 * (b.hi - b.lo);
     const double d = b.H - mu;
     o.rmsd2_L = 2.0 * (d > 0.0 ? d : 0.0);
     o.H = b.H;
-    o.iterations = 1;
     o.converged = b.pd & (half >= 0.0) & (half * half <= tol2L * o.rmsd2_L);
     return o;
 }
 
-// The sequence the Gram epilogue runs for a pair: one secular step from 0; a
-// second from the new lower bound while S is positive definite but the
-// bracket too wide; the quartic (qcp_shifted) when that still fails.
-// `steps` receives 1 or 2, or 0 when the quartic solved it.
-BS_HD ShiftedOut qcp_secular(const double* E, const double* ci, const double* cj, double tol2L,
-                             int* steps = nullptr)
+// The sequence the Gram epilogue runs for a pair: one secular step from 0,
+// a second from the new lower bound while S is positive definite but the
+// bracket too wide; a pair that still fails is left unconverged (the Gram
+// path re-evaluates it in FP64).  `steps` receives the steps taken.
+BS_HD PairSolve qcp_secular(const double* E, const double* ci, const double* cj, double tol2L,
+                            int* steps = nullptr)
 {
     const SecularSystem y = secular_system(E, ci, cj);
     SecularBracket b = secular_step(y, 0.0);
-    ShiftedOut o = secular_out(b, tol2L);
+    PairSolve o = secular_out(b, tol2L);
     int n = 1;
     if (!o.converged && b.pd) {
         b = secular_step(y, b.lo);
         o = secular_out(b, tol2L);
         n = 2;
     }
-    if (!o.converged) {
-        o = qcp_shifted(E, ci, cj);
-        n = 0;
-    }
     if (steps) *steps = n;
     return o;
-}
-
-// Two pairs sharing a column, solved in lockstep: coefficients, the first
-// Newton step and its verification are straight-line code for both, so the
-// two dependency chains interleave; a pair whose verification fails (rare)
-// continues alone.
-BS_HD void qcp_shifted2(const double* E0, const double* ci0, const double* E1, const double* ci1,
-                        const double* cj, ShiftedOut& o0, ShiftedOut& o1, int max_iter = 40)
-{
-    const ShiftedPoly s0 = shifted_poly(E0, ci0, cj);
-    const ShiftedPoly s1 = shifted_poly(E1, ci1, cj);
-    ShiftedIter t0{shifted_start(s0), 0.0, 0.0, 0, false, false, false, false};
-    ShiftedIter t1{shifted_start(s1), 0.0, 0.0, 0, false, false, false, false};
-    shifted_newton(s0, t0);
-    shifted_newton(s1, t1);
-    shifted_verify(s0, t0);
-    shifted_verify(s1, t1);
-    if (!t0.conv) shifted_continue(s0, t0, max_iter);
-    if (!t1.conv) shifted_continue(s1, t1, max_iter);
-    o0 = shifted_finish(s0, t0);
-    o1 = shifted_finish(s1, t1);
 }
 
 }  // namespace bs

@@ -6,17 +6,6 @@
 
 extern "C" {
 
-// out[0] = 2 (H - mu) (= L * RMSD^2), out[1] = H, out[2] = iterations,
-// out[3] = converged (1/0).
-void bsi_qcp_shifted(const double* E, const double* ci, const double* cj, double* out)
-{
-    const bs::ShiftedOut o = bs::qcp_shifted(E, ci, cj);
-    out[0] = o.rmsd2_L;
-    out[1] = o.H;
-    out[2] = o.iterations;
-    out[3] = o.converged ? 1.0 : 0.0;
-}
-
 double bsi_qcp_rmsd_f64(const double* M, double Ga, double Gb, int n)
 {
     return bs::qcp_rmsd<double>(M, Ga, Gb, n);
@@ -35,13 +24,13 @@ float bsi_qcp_rmsd_f32(const float* M, float Ga, float Gb, int n)
 
 extern "C" {
 
-// The secular-form solve with its fallbacks (the Gram epilogue's sequence):
-// out[0] = 2 (H - mu), out[1] = H, out[2] = secular steps taken (1, 2; 0 when
-// the quartic solved it), out[3] = converged (1/0).
+// The secular-form solve as the Gram epilogue runs it: out[0] = 2 (H - mu),
+// out[1] = H, out[2] = steps taken (1 or 2), out[3] = converged (1/0; an
+// unconverged pair is re-evaluated in FP64 on the device).
 void bsi_qcp_secular(const double* E, const double* ci, const double* cj, double tol2L, double* out)
 {
     int steps = 0;
-    const bs::ShiftedOut o = bs::qcp_secular(E, ci, cj, tol2L, &steps);
+    const bs::PairSolve o = bs::qcp_secular(E, ci, cj, tol2L, &steps);
     out[0] = o.rmsd2_L;
     out[1] = o.H;
     out[2] = steps;

@@ -161,38 +161,6 @@ def test_host_build_of_device_qcp_matches_oracle():
             assert f64(M.ctypes.data_as(dp), ga, gb, L) == pytest.approx(O.crmsd(a, b), abs=1e-9)
 
 
-def test_host_build_of_shifted_qcp_matches_oracle():
-    """The shifted-frame solve (residual Gram + per-conformation constants)."""
-    from oracle import oracle as O
-
-    dp = ctypes.POINTER(ctypes.c_double)
-    fn = _hook("bsi_qcp_shifted", None, [dp, dp, dp, dp])
-
-    def consts(e, r):
-        C, R0, g = r.T @ e, r.T @ r, (e * e).sum()
-        t0, tc = np.trace(R0), np.trace(C)
-        return np.array([
-            R0[0, 0] - t0 + 2 * (C[0, 0] - tc), R0[1, 1] - t0 + 2 * (C[1, 1] - tc),
-            R0[2, 2] - t0 + 2 * (C[2, 2] - tc), R0[0, 1] + C[0, 1] + C[1, 0],
-            R0[0, 2] + C[0, 2] + C[2, 0], R0[1, 2] + C[1, 2] + C[2, 1],
-            C[1, 2] - C[2, 1], C[2, 0] - C[0, 2], C[0, 1] - C[1, 0], g / 2, np.sqrt(g)])
-
-    x = B.generate_ensemble(3, 24, 50, sigma=(0.5, 4.0), near_fraction=0.2)
-    xc = np.array([O.centre(a)[0] for a in x])
-    # any reference works (the algebra is exact); use a noisy one, unaligned
-    ref = xc[0] + np.random.default_rng(1).normal(size=xc[0].shape)
-    ref -= ref.mean(0)
-    e = xc - ref
-    cs = [consts(ei, ref) for ei in e]
-    out = np.empty(4)
-    for i in range(24):
-        for j in range(i + 1, 24):
-            E = np.ascontiguousarray((e[i].T @ e[j]).ravel())
-            fn(E.ctypes.data_as(dp), cs[i].ctypes.data_as(dp), cs[j].ctypes.data_as(dp), out.ctypes.data_as(dp))
-            assert out[3] == 1.0
-            assert np.sqrt(out[0] / 50) == pytest.approx(O.crmsd(x[i], x[j]), abs=1e-8)
-
-
 def _kabsch(P, Q):
     """Rotation R minimising |P - Q R^T| for centred P, Q (numpy SVD)."""
     U, _, Vt = np.linalg.svd(Q.T @ P)
@@ -214,10 +182,12 @@ def _shift_consts(e, r):
                                              (12, (4.0, 8.0), True), (3, (0.5, 3.0), True), (50, (0.5, 4.0), False)])
 def test_host_build_of_secular_qcp_matches_oracle(L, sigma, aligned):
     """The Gram epilogue's solve: Newton on the secular form of the QCP
-    polynomial from mu = 0, a second step when the bracket is wide, the
-    quartic when S = -B is not positive definite.  Exact residual Grams (FP64)
+    polynomial from mu = 0, a second step when the bracket is wide; a pair it
+    cannot verify (S = -B not positive definite, or the bracket still wide) is
+    left unconverged for the FP64 re-evaluation.  Exact residual Grams (FP64)
     against the oracle's cRMSD; with conformations fitted onto their mean (the
-    device's frame) long chains converge in one step."""
+    device's frame) long chains converge in one step, short ones in two, and
+    with any reference every converged pair is right."""
     from oracle import oracle as O
 
     dp = ctypes.POINTER(ctypes.c_double)
@@ -236,7 +206,7 @@ def test_host_build_of_secular_qcp_matches_oracle(L, sigma, aligned):
     cs = [np.ascontiguousarray(_shift_consts(ei, ref)) for ei in e]
     tol2L = (2e-5 / 4) ** 2 * L
     out = np.empty(4)
-    steps = []
+    steps, conv = [], []
     for i in range(n):
         for j in range(i + 1, n):
             E = np.ascontiguousarray((e[i].T @ e[j]).ravel())
@@ -246,10 +216,14 @@ def test_host_build_of_secular_qcp_matches_oracle(L, sigma, aligned):
             want = O.crmsd(x[i], x[j])
             if out[3] == 1.0:
                 assert np.sqrt(out[0] / L) == pytest.approx(want, abs=2e-5 / 4 + 1e-9), (i, j, out)
-    steps = np.array(steps)
-    print(f"L={L}: one step {np.mean(steps == 1):.3f}, two {np.mean(steps == 2):.3f}, quartic {np.mean(steps == 0):.3f}")
+            conv.append(out[3] == 1.0)
+    steps, conv = np.array(steps), np.array(conv)
+    print(f"L={L}: one step {np.mean(steps == 1):.3f}, two {np.mean(steps == 2):.3f}, "
+          f"unconverged {np.mean(~conv):.3f}")
     if aligned and L >= 76:
-        assert np.all(steps == 1)
+        assert np.all(steps == 1) and conv.all()
+    if aligned and L >= 30:
+        assert conv.all()
 
 
 def test_host_build_of_fast_rotation():
