@@ -596,11 +596,21 @@ def main() -> None:
         torch.cuda.synchronize()
         plain_dt = (time.perf_counter() - t0) / args.plain_steps
         plain_kernel = [h["main_ms"] for h in ctx.history(args.plain_steps)]
+        pk_ms = float(np.mean(plain_kernel))
+        pk_tf = 18.0 * L * pairs / (pk_ms * 1e-3) / 1e12
         plain = {
             "method": "FP32_PLAIN",
             "value": round(pairs / plain_dt, 1),
             "ms_per_step": round(plain_dt * 1e3, 3),
-            "kernel_ms": round(float(np.mean(plain_kernel)), 3),
+            "kernel_ms": round(pk_ms, 3),
+            "kernel": "plain_kernel<float, 4> (4 x 4 pairs per thread: centring, covariance, QCP, sqrt per pair)",
+            "roofline": {
+                "bound": "fp32_valu", "achieved": round(pk_tf, 2), "peak": PEAK_FP32_VALU_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(pk_tf / PEAK_FP32_VALU_TFLOPS, 4),
+                "note": "algorithmic 18 L flops per pair (the covariance) over the kernel time; the peak "
+                        "counts packed FP32, which this build does not emit, so half of it is the "
+                        "ceiling for these FMAs",
+            },
         }
         log(f"fp32 plain: {plain_dt * 1e3:.3f} ms/step")
 

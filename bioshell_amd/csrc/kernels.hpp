@@ -15,10 +15,6 @@ namespace bs {
 void launch_plain_avsa(const double* xyz, int n, int L, bool fp64, void* out, bool out_f64,
                        hipStream_t s);
 
-// Query-vs-library variant of the plain kernel (dense n_q x n_lib output, f32).
-void launch_plain_qvl(const double* q_xyz, int n_q, const double* lib_xyz, int n_lib, int L,
-                      bool fp64, float* out, hipStream_t s);
-
 // ------------------------------------------------------------ Gram path
 enum GramMode : int { kGramBF16x3 = 0, kGramFP32 = 1 };
 

@@ -1,22 +1,28 @@
 // kernels_plain.hip -- the plain per-pair cRMSD kernel (FP64 and FP32).
 //
-// Each thread owns one pair and walks the textbook steps for it: the two
-// centred conformations (centroid = sum / n, then subtract, as in
+// Every pair walks the textbook steps: the two centred conformations
+// (centroid = sum / n, then subtract, as in
 // bioshell-pdb/src/calc/substructures.rs:27-42), the 3x3 cross-covariance
 // M = sum a b^T with the inner products Ga, Gb, the QCP quartic solved by
-// Newton from E0, and RMSD = sqrt(2 (E0 - lambda) / n).  A work-group covers
-// 16 x 16 pairs; the 32 conformations it touches are centred once per group
-// and streamed through LDS in chunks of 32 atoms.  This is the baseline form:
-// VALU-bound, 15 FMA per atom per pair.
+// Newton from E0, and RMSD = sqrt(2 (E0 - lambda) / n).
+//
+// Register-blocked for the VALU: a work-group of 256 threads covers a tile of
+// 16R x 16R pairs and each thread owns R x R of them (R = 4 in FP32, 2 in
+// FP64), rows ti R .. ti R + R-1 and columns tj R .. tj R + R-1 of the tile.
+// The tile's 32R conformations are centred once, then streamed through LDS
+// in chunks of 16 atoms, coordinate-major ([atom][coordinate][conformation]),
+// so that a thread reads its R rows' and R columns' value of one coordinate
+// as one R-wide vector: per atom 6 vector reads feed 9 R^2 multiply-adds
+// (and 6 R for Ga, Gb).  The pairs' matrices then go through LDS in rounds,
+// and one copy of the QCP solve serves every pair of a thread.
 #include "kernels.hpp"
 #include "qcp.hpp"
 
 namespace bs {
 namespace {
 
-constexpr int kTile = 16;
-constexpr int kChunk = 32;
-constexpr int kRowStride = kChunk * 3 + 1;  // +1 keeps the 16 rows in distinct banks
+constexpr int kChunk = 16;  // atoms per LDS chunk
+constexpr int kThreads = 256;
 
 __device__ inline int64_t tri_index(int64_t n, int64_t i, int64_t j)
 {
@@ -35,44 +41,46 @@ __device__ inline void tri_tile(int b, int nt, int& bi, int& bj)
     bj = r + (b - start(r));
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void plain_kernel(const double* __restrict__ rows_xyz,
-                                                    const double* __restrict__ cols_xyz,
-                                                    int n_rows, int n_cols, int L, int triangle,
-                                                    int nt, int64_t out_ld, void* __restrict__ out,
-                                                    int out_f64)
+template <class T, int R>
+struct PlainShape {
+    static constexpr int kTile = 16 * R;               // conformations per tile side
+    static constexpr int kConfs = 2 * kTile;           // rows then columns
+    static constexpr int kLd = kConfs + 4;             // LDS row stride (floats or doubles)
+    static constexpr int kPairs = R * R;               // pairs per thread
+    static constexpr int kRound = R == 4 ? 4 : 2;      // pairs per thread per stash round
+    static constexpr int kStage = kChunk * 3 * kLd;    // staged coordinates (elements)
+    static constexpr int kStash = kRound * 11 * kThreads;
+    static constexpr int kLds = kStage > kStash ? kStage : kStash;
+};
+
+template <class T, int R>
+__global__ __launch_bounds__(kThreads) void plain_kernel(const double* __restrict__ xyz, int n, int L,
+                                                         int nt, void* __restrict__ out, int out_f64)
 {
-    __shared__ T cen[2 * kTile][3];
-    __shared__ T sa[kTile][kRowStride];
-    __shared__ T sb[kTile][kRowStride];
+    using S = PlainShape<T, R>;
+    __shared__ T cen[S::kConfs][3];
+    __shared__ T buf[S::kLds];
 
     int bi, bj;
-    if (triangle) {
-        tri_tile(blockIdx.x, nt, bi, bj);
-    } else {
-        bi = blockIdx.y;
-        bj = blockIdx.x;
-    }
+    tri_tile(blockIdx.x, nt, bi, bj);
     const int tid = threadIdx.x;
 
-    // ---- step 1: centroids of the 16 row and 16 column conformations
+    // ---- step 1: centroids of the tile's conformations (sum, then divide)
     {
-        const int c = tid >> 3, part = tid & 7;
-        const bool is_row = c < kTile;
-        const int conf = is_row ? bi * kTile + c : bj * kTile + (c - kTile);
-        const int nconf = is_row ? n_rows : n_cols;
-        const double* src = is_row ? rows_xyz : cols_xyz;
+        constexpr int per = kThreads / S::kConfs;  // threads per conformation
+        const int c = tid / per, part = tid % per;
+        const int conf = c < S::kTile ? bi * S::kTile + c : bj * S::kTile + (c - S::kTile);
         T s0 = T(0), s1 = T(0), s2 = T(0);
-        if (conf < nconf) {
-            const double* p = src + (int64_t)conf * L * 3;
-            for (int k = part; k < L; k += 8) {
+        if (conf < n) {
+            const double* p = xyz + (int64_t)conf * L * 3;
+            for (int k = part; k < L; k += per) {
                 s0 += T(p[3 * k + 0]);
                 s1 += T(p[3 * k + 1]);
                 s2 += T(p[3 * k + 2]);
             }
         }
 #pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
+        for (int o = 1; o < per; o <<= 1) {
             s0 += __shfl_xor(s0, o);
             s1 += __shfl_xor(s1, o);
             s2 += __shfl_xor(s2, o);
@@ -85,53 +93,94 @@ __global__ __launch_bounds__(256) void plain_kernel(const double* __restrict__ r
     }
     __syncthreads();
 
-    // ---- step 2: covariance of the centred sets, chunk by chunk
-    const int ti = tid / kTile, tj = tid % kTile;
-    T m[9];
+    // ---- step 2: covariances of the thread's R x R pairs, chunk by chunk
+    const int ti = tid / 16, tj = tid % 16;
+    T m[R][R][9];
+    T ga[R], gb[R];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) m[q] = T(0);
-    T ga = T(0), gb = T(0);
-
+    for (int u = 0; u < R; ++u) {
+        ga[u] = gb[u] = T(0);
+#pragma unroll
+        for (int v = 0; v < R; ++v)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) m[u][v][q] = T(0);
+    }
     for (int k0 = 0; k0 < L; k0 += kChunk) {
         const int nk = min(kChunk, L - k0);
-        for (int v = tid; v < 2 * kTile * kChunk * 3; v += 256) {
-            const int half = v / (kTile * kChunk * 3);
-            const int w = v - half * (kTile * kChunk * 3);
-            const int r = w / (kChunk * 3), rem = w - r * (kChunk * 3);
-            const int conf = (half == 0 ? bi : bj) * kTile + r;
-            const int nconf = half == 0 ? n_rows : n_cols;
-            const double* src = half == 0 ? rows_xyz : cols_xyz;
+        // coalesced reads of each conformation's chunk (3 nk consecutive
+        // doubles), centred, stored coordinate-major
+        for (int v = tid; v < S::kConfs * kChunk * 3; v += kThreads) {
+            const int c = v / (kChunk * 3), rem = v - c * (kChunk * 3);
+            const int conf = c < S::kTile ? bi * S::kTile + c : bj * S::kTile + (c - S::kTile);
             T val = T(0);
-            if (conf < nconf && rem < nk * 3)
-                val = T(src[((int64_t)conf * L + k0) * 3 + rem]) - cen[half * kTile + r][rem % 3];
-            if (half == 0)
-                sa[r][rem] = val;
-            else
-                sb[r][rem] = val;
+            if (conf < n && rem < nk * 3)
+                val = T(xyz[((int64_t)conf * L + k0) * 3 + rem]) - cen[c][rem % 3];
+            buf[rem * S::kLd + c] = val;
         }
         __syncthreads();
         for (int kk = 0; kk < nk; ++kk) {
-            const T a0 = sa[ti][3 * kk], a1 = sa[ti][3 * kk + 1], a2 = sa[ti][3 * kk + 2];
-            const T b0 = sb[tj][3 * kk], b1 = sb[tj][3 * kk + 1], b2 = sb[tj][3 * kk + 2];
-            m[0] += a0 * b0; m[1] += a0 * b1; m[2] += a0 * b2;
-            m[3] += a1 * b0; m[4] += a1 * b1; m[5] += a1 * b2;
-            m[6] += a2 * b0; m[7] += a2 * b1; m[8] += a2 * b2;
-            ga += a0 * a0 + a1 * a1 + a2 * a2;
-            gb += b0 * b0 + b1 * b1 + b2 * b2;
+            T a[3][R], b[3][R];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    a[c][u] = buf[(3 * kk + c) * S::kLd + ti * R + u];
+                    b[c][u] = buf[(3 * kk + c) * S::kLd + S::kTile + tj * R + u];
+                }
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                ga[u] += a[0][u] * a[0][u] + a[1][u] * a[1][u] + a[2][u] * a[2][u];
+                gb[u] += b[0][u] * b[0][u] + b[1][u] * b[1][u] + b[2][u] * b[2][u];
+            }
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+#pragma unroll
+                for (int v = 0; v < R; ++v)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) m[u][v][3 * c + d] += a[c][u] * b[d][v];
         }
         __syncthreads();
     }
 
-    // ---- steps 3-4: QCP eigenvalue and RMSD
-    const int i = bi * kTile + ti, j = bj * kTile + tj;
-    if (i >= n_rows || j >= n_cols) return;
-    if (triangle && i >= j) return;
-    const T r = qcp_rmsd<T>(m, ga, gb, L);
-    const int64_t o = triangle ? tri_index(n_rows, i, j) : (int64_t)i * out_ld + j;
-    if (out_f64)
-        static_cast<double*>(out)[o] = double(r);
-    else
-        static_cast<float*>(out)[o] = float(r);
+    // ---- steps 3-4: QCP eigenvalue and RMSD, one pair at a time through LDS
+    T* stash = buf;  // [pair slot][value][thread]
+#pragma unroll
+    for (int r0 = 0; r0 < S::kPairs; r0 += S::kRound) {
+#pragma unroll
+        for (int p = 0; p < S::kRound; ++p) {
+            const int u = (r0 + p) / R, v = (r0 + p) % R;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) stash[(p * 11 + q) * kThreads + tid] = m[u][v][q];
+            stash[(p * 11 + 9) * kThreads + tid] = ga[u];
+            stash[(p * 11 + 10) * kThreads + tid] = gb[v];
+        }
+#pragma unroll 1
+        for (int p = 0; p < S::kRound; ++p) {  // each thread reads back its own slots
+            const int u = (r0 + p) / R, v = (r0 + p) % R;
+            const int i = bi * S::kTile + ti * R + u, j = bj * S::kTile + tj * R + v;
+            T mm[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) mm[q] = stash[(p * 11 + q) * kThreads + tid];
+            const T gi = stash[(p * 11 + 9) * kThreads + tid], gj = stash[(p * 11 + 10) * kThreads + tid];
+            if (i < n && j < n && i < j) {
+                const T r = qcp_rmsd<T>(mm, gi, gj, L);
+                const int64_t o = tri_index(n, i, j);
+                if (out_f64)
+                    static_cast<double*>(out)[o] = double(r);
+                else
+                    static_cast<float*>(out)[o] = float(r);
+            }
+        }
+    }
+}
+
+template <class T, int R>
+void launch(const double* xyz, int n, int L, void* out, bool out_f64, hipStream_t s)
+{
+    const int nt = (n + PlainShape<T, R>::kTile - 1) / PlainShape<T, R>::kTile;
+    plain_kernel<T, R><<<nt * (nt + 1) / 2, kThreads, 0, s>>>(xyz, n, L, nt, out, out_f64 ? 1 : 0);
 }
 
 }  // namespace
@@ -140,25 +189,10 @@ void launch_plain_avsa(const double* xyz, int n, int L, bool fp64, void* out, bo
                        hipStream_t s)
 {
     if (n < 2) return;
-    const int nt = (n + kTile - 1) / kTile;
-    const int blocks = nt * (nt + 1) / 2;
     if (fp64)
-        plain_kernel<double><<<blocks, 256, 0, s>>>(xyz, xyz, n, n, L, 1, nt, 0, out, out_f64);
+        launch<double, 2>(xyz, n, L, out, out_f64, s);
     else
-        plain_kernel<float><<<blocks, 256, 0, s>>>(xyz, xyz, n, n, L, 1, nt, 0, out, out_f64);
-}
-
-void launch_plain_qvl(const double* q_xyz, int n_q, const double* lib_xyz, int n_lib, int L,
-                      bool fp64, float* out, hipStream_t s)
-{
-    if (n_q < 1 || n_lib < 1) return;
-    dim3 grid((n_lib + kTile - 1) / kTile, (n_q + kTile - 1) / kTile);
-    if (fp64)
-        plain_kernel<double><<<grid, 256, 0, s>>>(q_xyz, lib_xyz, n_q, n_lib, L, 0, 0, n_lib, out,
-                                                  0);
-    else
-        plain_kernel<float><<<grid, 256, 0, s>>>(q_xyz, lib_xyz, n_q, n_lib, L, 0, 0, n_lib, out,
-                                                 0);
+        launch<float, 4>(xyz, n, L, out, out_f64, s);
 }
 
 }  // namespace bs
