@@ -85,7 +85,7 @@ struct Plan {
     std::vector<int> src;  // caller's index of each planned ensemble (n >= 2)
     std::vector<int> n, L, n_pad, n_kblk, tile, n_ref, slices;
     std::vector<int64_t> xyz_off, ref_off, part_off, pan_off, con_off, out_off;
-    std::vector<int> conf_prefix, samp_prefix, pad_prefix, slice_prefix, coord_prefix;
+    std::vector<int> conf_prefix, fit_prefix, slice_prefix, coord_prefix;
     std::vector<int64_t> word_prefix;
     std::vector<uint32_t> items;
     int list_start[kXcdLists + 1] = {};
@@ -107,19 +107,18 @@ Plan make_plan(const bs_context* ctx, bs_method m, const size_t* n_conf, const s
         if (n_conf[e] >= 2) P.src.push_back((int)e);
     const int E = (int)P.src.size();
     P.conf_prefix.assign(E + 1, 0);
-    P.samp_prefix.assign(E + 1, 0);
-    P.pad_prefix.assign(E + 1, 0);
+    P.fit_prefix.assign(E + 1, 0);
     P.slice_prefix.assign(E + 1, 0);
     P.coord_prefix.assign(E + 1, 0);
     P.word_prefix.assign(E + 1, 0);
-    int64_t conf_total = 0, samp_total = 0, pad_total = 0, slice_total = 0, coord_total = 0;
+    int64_t conf_total = 0, pad_total = 0, fit_total = 0, slice_total = 0, coord_total = 0;
     std::vector<double> cost(E, 0.0);
     for (int e = 0; e < E; ++e) {
         const int n = (int)n_conf[P.src[e]], L = (int)n_atoms[P.src[e]];
         const int tile = force_tile ? force_tile : choose_tile(ctx, m, n, L);
         const int n_pad = round_up(n, tile), n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
         const int n_ref = ref_sample_count(n);
-        const int slices = (n_ref + kMeanSliceConfs - 1) / kMeanSliceConfs;
+        const int slices = ref_slices(n_ref);
         const int64_t pairs = (int64_t)n * (n - 1) / 2;
         P.n.push_back(n);
         P.L.push_back(L);
@@ -143,15 +142,14 @@ Plan make_plan(const bs_context* ctx, bs_method m, const size_t* n_conf, const s
         P.out_values += pairs;
         P.pairs += (uint64_t)pairs;
         conf_total += n;
-        samp_total += n_ref;
         pad_total += n_pad;
+        fit_total += fit_groups(n_pad);
         slice_total += slices;
         coord_total += 3 * L;
         BS_REQUIRE(conf_total < (1ll << 30) && pad_total < (1ll << 30) && coord_total < (1ll << 30),
                    "batch too large");
         P.conf_prefix[e + 1] = (int)conf_total;
-        P.samp_prefix[e + 1] = (int)samp_total;
-        P.pad_prefix[e + 1] = (int)pad_total;
+        P.fit_prefix[e + 1] = (int)fit_total;
         P.slice_prefix[e + 1] = (int)slice_total;
         P.coord_prefix[e + 1] = (int)coord_total;
         P.word_prefix[e + 1] = P.word_prefix[e] + (int64_t)align_up((size_t)(pairs + 31) / 32, 4);
@@ -224,6 +222,68 @@ size_t put(std::vector<unsigned char>& blob, const void* src, size_t bytes)
     return off;
 }
 
+// The batch's descriptors on the device: blob goes up (through the pinned
+// staging buffer, on stream s) unless the device copy already holds it.
+char* upload_meta(BatchWorkspace& W, std::vector<unsigned char>& blob, hipStream_t s)
+{
+    const size_t meta_bytes = align_up(blob.size(), 256);
+    blob.resize(meta_bytes);
+    const void* meta_before = W.meta.get();
+    char* meta = static_cast<char*>(W.meta.ensure(meta_bytes, "batch metadata"));
+    if (meta != meta_before || meta != W.meta_last_dev || blob != W.meta_last) {
+        if (!W.meta_done) BS_HIP(hipEventCreateWithFlags(&W.meta_done, hipEventDisableTiming));
+        BS_HIP(hipEventSynchronize(W.meta_done));  // the previous upload has left the pinned buffer
+        void* h = ensure_pinned(W.h_meta, W.h_meta_cap, meta_bytes);
+        std::memcpy(h, blob.data(), meta_bytes);
+        BS_HIP(hipMemcpyAsync(meta, h, meta_bytes, hipMemcpyHostToDevice, s));
+        BS_HIP(hipEventRecord(W.meta_done, s));
+        W.meta_last.swap(blob);
+        W.meta_last_dev = meta;
+    }
+    return meta;
+}
+
+// The per-pair kernel over the batch: the ensembles grouped by tile block
+// (plain_block), one launch per group over all of its tiles.
+void enqueue_plain(bs_context* ctx, const Plan& P, const std::vector<const double*>& d_xyz,
+                   const std::vector<float*>& d_out, RunRecord* rec)
+{
+    hipStream_t s = ctx->stream;
+    const bool fp64 = P.method == BS_FP64_PLAIN;
+    const int E = (int)P.src.size();
+    std::vector<PlainEns> grp[3];  // blocks 1, 2, 4
+    std::vector<int> pre[3];
+    for (int g = 0; g < 3; ++g) pre[g].push_back(0);
+    for (int e = 0; e < E; ++e) {
+        const int r = plain_block(P.n[e], fp64, false, ctx->n_cu);
+        const int g = r == 1 ? 0 : r == 2 ? 1 : 2;
+        const int nt = plain_tiles_per_side(P.n[e], r);
+        const int64_t total = (int64_t)pre[g].back() + (int64_t)nt * (nt + 1) / 2;
+        BS_REQUIRE(total < (1ll << 31), "batch too large for the per-pair kernel");
+        grp[g].push_back({d_xyz[e], d_out[e], P.n[e], P.L[e], nt, 0});
+        pre[g].push_back((int)total);
+    }
+    std::vector<unsigned char> blob;
+    size_t o_ens[3], o_pre[3];
+    for (int g = 0; g < 3; ++g) {
+        o_ens[g] = put(blob, grp[g].data(), grp[g].size() * sizeof(PlainEns));
+        o_pre[g] = put(blob, pre[g].data(), pre[g].size() * 4);
+    }
+    char* meta = upload_meta(ctx->batch, blob, s);
+    rec->mark({kBTotal0, kBPrep0, kBPrep1, kBMain0}, s);
+    uint64_t launches = 0;
+    for (int g = 0; g < 3; ++g) {
+        if (grp[g].empty()) continue;
+        launch_plain_batch(reinterpret_cast<const PlainEns*>(meta + o_ens[g]),
+                           reinterpret_cast<const int*>(meta + o_pre[g]), (int)grp[g].size(),
+                           pre[g].back(), 1 << g, fp64, s);
+        ++launches;
+    }
+    BS_HIP(hipGetLastError());
+    rec->mark({kBMain1, kBFallback0, kBFallback1, kBTotal1}, s);
+    rec->base.main_launches = launches;
+}
+
 // Enqueues the batch on the context's stream.  d_xyz[e] / d_out[e] are the
 // device input and output of planned ensemble e.
 void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d_xyz,
@@ -241,12 +301,7 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
         return;
     }
     if (!P.gram) {
-        rec->mark({kBTotal0, kBPrep0, kBPrep1, kBMain0}, s);
-        for (int e = 0; e < E; ++e)
-            launch_plain_avsa(d_xyz[e], P.n[e], P.L[e], P.method == BS_FP64_PLAIN, d_out[e], false, s);
-        BS_HIP(hipGetLastError());
-        rec->mark({kBMain1, kBFallback0, kBFallback1, kBTotal1}, s);
-        rec->base.main_launches = (uint64_t)E;
+        enqueue_plain(ctx, P, d_xyz, d_out, rec);
         return;
     }
 
@@ -288,8 +343,7 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     std::vector<unsigned char> blob;
     blob.reserve(4096);
     const size_t o_ens = put(blob, ens.data(), ens.size() * sizeof(BatchEns));
-    const size_t o_cp = put(blob, P.samp_prefix.data(), P.samp_prefix.size() * 4);
-    const size_t o_pp = put(blob, P.pad_prefix.data(), P.pad_prefix.size() * 4);
+    const size_t o_pp = put(blob, P.fit_prefix.data(), P.fit_prefix.size() * 4);
     const size_t o_sp = put(blob, P.slice_prefix.data(), P.slice_prefix.size() * 4);
     const size_t o_kp = put(blob, P.coord_prefix.data(), P.coord_prefix.size() * 4);
     const size_t o_wp = put(blob, P.word_prefix.data(), P.word_prefix.size() * 8);
@@ -297,33 +351,17 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     const size_t o_it = put(blob, P.items.data(), P.items.size() * 4);
     const size_t o_64 = put(blob, P.t64.data(), P.t64.size() * 8);
     const size_t o_32 = put(blob, P.t32.data(), P.t32.size() * 8);
-    const size_t meta_bytes = align_up(blob.size(), 256);
-    blob.resize(meta_bytes);
-
-    const void* meta_before = W.meta.get();
-    char* meta = static_cast<char*>(W.meta.ensure(meta_bytes, "batch metadata"));
-    if (meta != meta_before || meta != W.meta_last_dev || blob != W.meta_last) {
-        if (!W.meta_done) BS_HIP(hipEventCreateWithFlags(&W.meta_done, hipEventDisableTiming));
-        BS_HIP(hipEventSynchronize(W.meta_done));  // the previous upload has left the pinned buffer
-        void* h = ensure_pinned(W.h_meta, W.h_meta_cap, meta_bytes);
-        std::memcpy(h, blob.data(), meta_bytes);
-        BS_HIP(hipMemcpyAsync(meta, h, meta_bytes, hipMemcpyHostToDevice, s));
-        BS_HIP(hipEventRecord(W.meta_done, s));
-        W.meta_last.swap(blob);
-        W.meta_last_dev = meta;
-    }
+    char* meta = upload_meta(W, blob, s);
 
     BatchArgs a{};
     a.ens = reinterpret_cast<const BatchEns*>(meta + o_ens);
     a.n_ens = E;
     a.mode = P.mode;
-    a.samp_prefix = reinterpret_cast<const int*>(meta + o_cp);
-    a.pad_prefix = reinterpret_cast<const int*>(meta + o_pp);
+    a.fit_prefix = reinterpret_cast<const int*>(meta + o_pp);
     a.slice_prefix = reinterpret_cast<const int*>(meta + o_sp);
     a.coord_prefix = reinterpret_cast<const int*>(meta + o_kp);
     a.word_prefix = reinterpret_cast<const int64_t*>(meta + o_wp);
-    a.total_samp = P.samp_prefix[E];
-    a.total_pad = P.pad_prefix[E];
+    a.total_fit = P.fit_prefix[E];
     a.total_slices = P.slice_prefix[E];
     a.total_coords = P.coord_prefix[E];
     a.total_words = total_words;

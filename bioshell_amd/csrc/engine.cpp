@@ -17,7 +17,7 @@ PreparedSizes prepared_sizes(int n, int L)
 {
     const size_t n_pad = (size_t)round_up(n, kTileRows);
     const size_t n_kblk = (size_t)(L + kAtomBlock - 1) / kAtomBlock;
-    const size_t slices = (size_t)(ref_sample_count(n) + kMeanSliceConfs - 1) / kMeanSliceConfs;
+    const size_t slices = (size_t)ref_slices(ref_sample_count(n));
     PreparedSizes z;
     z.cent = (size_t)n * 3 * 8;
     z.rot = (size_t)n * 9 * 8;
@@ -55,7 +55,7 @@ PrepView prepare_into(const PreparedPtrs& p, const double* d_xyz, int n, int L, 
     const int n_pad = round_up(n, kTileRows), n_kblk = (L + kAtomBlock - 1) / kAtomBlock;
     const double* ref = fixed_ref;  // a fixed reference is read in place
     if (!fixed_ref) {
-        launch_reference(d_xyz, n, L, p.rot, p.cent, p.partial, p.ref, s);
+        launch_reference(d_xyz, n, L, p.partial, p.ref, s);
         ref = p.ref;
     }
     launch_fit_pack(d_xyz, ref, n, n_pad, L, n_kblk, mode, p.rot, p.cent, p.panels, p.consts, n_pad, s);
@@ -317,7 +317,7 @@ void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m
     rec->base.pairs = n >= 2 ? (uint64_t)n * (n - 1) / 2 : 0;
     if (n >= 2 && !is_gram(m)) {
         rec->mark({kBTotal0, kBPrep0, kBPrep1, kBMain0}, s);
-        launch_plain_avsa(d_xyz, n, L, m == BS_FP64_PLAIN, d_out, out_f64, s);
+        launch_plain_avsa(d_xyz, n, L, m == BS_FP64_PLAIN, d_out, out_f64, ctx->n_cu, s);
         BS_HIP(hipGetLastError());
         rec->mark({kBMain1, kBFallback0, kBFallback1, kBTotal1}, s);
         rec->base.main_launches = 1;

@@ -12,8 +12,24 @@ namespace bs {
 // Per-pair kernel over the upper triangle of one ensemble: centre, 3x3
 // covariance, textbook QCP, sqrt -- in T = double (fp64 != 0) or float.
 // xyz: device, n x L x 3 f64 AoS.  out: packed triangle, f64 or f32.
+// The tile side is 16 R conformations, R = 1, 2 or 4 (fp32) and 1 or 2
+// (fp64), chosen by plain_block from the pair slots the tiles cover and, for
+// a lone launch (alone), from how many rounds of 2 work-groups per CU they
+// take on n_cu CUs.
 void launch_plain_avsa(const double* xyz, int n, int L, bool fp64, void* out, bool out_f64,
-                       hipStream_t s);
+                       int n_cu, hipStream_t s);
+int plain_block(int n, bool fp64, bool alone, int n_cu);
+inline int plain_tiles_per_side(int n, int r) { return (n + 16 * r - 1) / (16 * r); }
+// One ensemble of a batched plain launch (f32 output); nt tiles per side.
+struct PlainEns {
+    const double* xyz;
+    void* out;
+    int n, L, nt, pad;
+};
+// Every ensemble of ens[0 .. n_ens) with block r in one launch: prefix (n_ens + 1
+// entries, device) counts their tiles, tiles = prefix[n_ens].
+void launch_plain_batch(const PlainEns* ens, const int* prefix, int n_ens, int tiles, int r, bool fp64,
+                        hipStream_t s);
 
 // ------------------------------------------------------------ Gram path
 enum GramMode : int { kGramBF16x3 = 0, kGramFP32 = 1 };
@@ -30,7 +46,13 @@ constexpr int kPanelBlockU4 = 384;  // 16-byte units per (conf block, atom block
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-constexpr int kMeanSliceConfs = 8;  // sampled conformations per partial sum of the reference mean
+// Per-conformation fits (prep_device.hpp): kFitConfs conformations per
+// work-group of 256 threads, kFitLanes threads each.
+constexpr int kFitConfs = 16;
+constexpr int kFitLanes = 256 / kFitConfs;
+__host__ __device__ inline int fit_groups(int count) { return (count + kFitConfs - 1) / kFitConfs; }
+// The reference mean's partial sums: one per work-group of sample fits.
+__host__ __device__ inline int ref_slices(int S) { return fit_groups(S); }
 
 // The reference r of the shifted frame is the Procrustes mean of S(n)
 // conformations spread evenly over the ensemble: sample s is conformation
@@ -53,12 +75,9 @@ __host__ __device__ inline int ref_sample_index(int s, int n, int S)
 }
 
 // Reference of one ensemble (prep_device.hpp): for the S = ref_sample_count(n)
-// samples, centroids (cent, S x 3) and rotations onto conformation 0 (rot,
-// S x 9) as by-products, the slices' partial sums (partial, ceil(S/8) x 3L),
-// then r (ref: 3L values) and R0 (6).  cent and rot hold n entries; the fit
-// onto r overwrites them.
-void launch_reference(const double* xyz, int n, int L, double* rot, double* cent, double* partial,
-                      double* ref, hipStream_t s);
+// samples, the slices' partial sums of the fitted samples (partial,
+// ref_slices(S) x 3L), then r (ref: 3L values) and R0 (6).
+void launch_reference(const double* xyz, int n, int L, double* partial, double* ref, hipStream_t s);
 // Fit onto ref and pack: rotations (rot), centroids (cent), operand panels of
 // e_i = rot_i (x_i - c_i) - ref and the constants, for n_pad rows.
 void launch_fit_pack(const double* xyz, const double* ref, int n, int n_pad, int L, int n_kblk,
@@ -159,12 +178,11 @@ struct BatchArgs {
     const BatchEns* ens;
     int n_ens;
     int mode;
-    const int* samp_prefix;  // reference-mean samples
-    const int* pad_prefix;
+    const int* fit_prefix;   // fit + pack work-groups (fit_groups of n_pad)
     const int* slice_prefix;
     const int* coord_prefix;
     const int64_t* word_prefix;
-    int total_samp, total_pad, total_slices, total_coords;
+    int total_fit, total_slices, total_coords;
     int64_t total_words;
     const uint32_t* items;
     const int* list_start;  // 9 entries: list x is items[list_start[x] .. list_start[x+1])

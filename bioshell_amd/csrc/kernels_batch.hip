@@ -33,28 +33,14 @@ __device__ inline int find_ens(const T* __restrict__ prefix, int n_ens, T g)
 }
 
 // ------------------------------------------------------------------ prep
+// One work-group per slice of the reference mean (fit0_slice).
 __global__ __launch_bounds__(256) void batch_fit0_kernel(BatchArgs b)
 {
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= b.total_samp) return;
-    const int e = find_ens(b.samp_prefix, b.n_ens, g);
-    const BatchEns& E = b.ens[e];
-    const int t = g - b.samp_prefix[e];
-    const int i = ref_sample_index(t, E.n, E.n_ref);
-    fit0_one(E.xyz + (int64_t)i * E.L * 3, E.xyz, E.L, E.rot + (int64_t)t * 9,
-             E.cent + (int64_t)t * 3, threadIdx.x & 63);
-}
-
-__global__ __launch_bounds__(256) void batch_mean_partial_kernel(BatchArgs b)
-{
-    __shared__ double sR[kMeanSliceConfs][9];
-    __shared__ double sC[kMeanSliceConfs][3];
-    __shared__ int64_t sX[kMeanSliceConfs];
+    __shared__ Fit0Lds lds;
     const int g = blockIdx.x;
     const int e = find_ens(b.slice_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    mean_partial_block(E.xyz, E.cent, E.rot, E.n, E.n_ref, E.L, g - b.slice_prefix[e], E.partial,
-                       sR, sC, sX);
+    fit0_slice(E.xyz, E.n, E.n_ref, E.L, g - b.slice_prefix[e], E.partial, lds);
 }
 
 // One thread per reference coordinate, as in the single-ensemble kernel.
@@ -77,17 +63,13 @@ __global__ __launch_bounds__(256) void batch_ref_gram_kernel(BatchArgs b)
 template <int MODE>
 __global__ __launch_bounds__(256) void batch_fit_pack_kernel(BatchArgs b)
 {
-    __shared__ uint4 img[4][kPackWordsU4];
-    const int wv = threadIdx.x >> 6;
-    const int blk = xcd_contiguous_block((b.total_pad + 3) / 4);
-    if (blk < 0) return;
-    const int g = blk * 4 + wv;
-    if (g >= b.total_pad) return;
-    const int e = find_ens(b.pad_prefix, b.n_ens, g);
+    __shared__ FitPackLds lds;
+    const int g = xcd_contiguous_block(b.total_fit);
+    if (g < 0) return;
+    const int e = find_ens(b.fit_prefix, b.n_ens, g);
     const BatchEns& E = b.ens[e];
-    fit_pack_one<MODE>(E.xyz, E.ref, E.n, E.L, E.n_kblk, g - b.pad_prefix[e], E.rot, E.cent,
-                       static_cast<uint4*>(E.panels), E.consts, E.n_pad, img[wv],
-                       threadIdx.x & 63);
+    fit_pack_block<MODE>(E.xyz, E.ref, E.n, E.L, E.n_kblk, (g - b.fit_prefix[e]) * kFitConfs, E.rot,
+                         E.cent, static_cast<uint4*>(E.panels), E.consts, E.n_pad, lds);
 }
 
 __global__ void batch_zero_kernel(uint4* p, int64_t n4)
@@ -253,12 +235,10 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(BatchArgs b)
 
 void launch_batch_prep(const BatchArgs& b, hipStream_t s)
 {
-    const int cw = (b.total_samp + 3) / 4;
-    batch_fit0_kernel<<<cw, 256, 0, s>>>(b);
-    batch_mean_partial_kernel<<<b.total_slices, 256, 0, s>>>(b);
+    batch_fit0_kernel<<<b.total_slices, 256, 0, s>>>(b);
     batch_mean_final_kernel<<<(b.total_coords + 255) / 256, 256, 0, s>>>(b);
     batch_ref_gram_kernel<<<b.n_ens, 256, 0, s>>>(b);
-    const int pw = xcd_contiguous_grid((b.total_pad + 3) / 4);
+    const int pw = xcd_contiguous_grid(b.total_fit);
     if (b.mode == kGramBF16x3)
         batch_fit_pack_kernel<kGramBF16x3><<<pw, 256, 0, s>>>(b);
     else

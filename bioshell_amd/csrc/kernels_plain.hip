@@ -18,6 +18,8 @@
 #include "kernels.hpp"
 #include "qcp.hpp"
 
+#include <cmath>
+
 namespace bs {
 namespace {
 
@@ -47,22 +49,20 @@ struct PlainShape {
     static constexpr int kConfs = 2 * kTile;           // rows then columns
     static constexpr int kLd = kConfs + 4;             // LDS row stride (floats or doubles)
     static constexpr int kPairs = R * R;               // pairs per thread
-    static constexpr int kRound = R == 4 ? 4 : 2;      // pairs per thread per stash round
+    static constexpr int kRound = R == 4 ? 4 : R == 2 ? 2 : 1;  // pairs per thread per stash round
     static constexpr int kStage = kChunk * 3 * kLd;    // staged coordinates (elements)
     static constexpr int kStash = kRound * 11 * kThreads;
     static constexpr int kLds = kStage > kStash ? kStage : kStash;
 };
 
+// Tile (bi, bj) of one ensemble.
 template <class T, int R>
-__global__ __launch_bounds__(kThreads) void plain_kernel(const double* __restrict__ xyz, int n, int L,
-                                                         int nt, void* __restrict__ out, int out_f64)
+__device__ __forceinline__ void plain_tile(const double* __restrict__ xyz, int n, int L, int bi, int bj,
+                                  void* __restrict__ out, int out_f64)
 {
     using S = PlainShape<T, R>;
     __shared__ T cen[S::kConfs][3];
     __shared__ T buf[S::kLds];
-
-    int bi, bj;
-    tri_tile(blockIdx.x, nt, bi, bj);
     const int tid = threadIdx.x;
 
     // ---- step 1: centroids of the tile's conformations (sum, then divide)
@@ -177,22 +177,110 @@ __global__ __launch_bounds__(kThreads) void plain_kernel(const double* __restric
 }
 
 template <class T, int R>
+__global__ __launch_bounds__(kThreads) void plain_kernel(const double* __restrict__ xyz, int n, int L,
+                                                         int nt, void* __restrict__ out, int out_f64)
+{
+    int bi, bj;
+    tri_tile(blockIdx.x, nt, bi, bj);
+    plain_tile<T, R>(xyz, n, L, bi, bj, out, out_f64);
+}
+
+// Many ensembles in one launch: block b is tile b - prefix[e] of ensemble e,
+// prefix[e] <= b < prefix[e + 1] (f32 output).
+template <class T, int R>
+__global__ __launch_bounds__(kThreads) void plain_batch_kernel(const PlainEns* __restrict__ ens,
+                                                               const int* __restrict__ prefix, int n_ens)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = n_ens;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const PlainEns E = ens[lo];
+    int bi, bj;
+    tri_tile(b - prefix[lo], E.nt, bi, bj);
+    plain_tile<T, R>(E.xyz, E.n, E.L, bi, bj, E.out, 0);
+}
+
+template <class T, int R>
 void launch(const double* xyz, int n, int L, void* out, bool out_f64, hipStream_t s)
 {
-    const int nt = (n + PlainShape<T, R>::kTile - 1) / PlainShape<T, R>::kTile;
+    const int nt = plain_tiles_per_side(n, R);
     plain_kernel<T, R><<<nt * (nt + 1) / 2, kThreads, 0, s>>>(xyz, n, L, nt, out, out_f64 ? 1 : 0);
+}
+
+template <class T, int R>
+void launch_batch(const PlainEns* ens, const int* prefix, int n_ens, int tiles, hipStream_t s)
+{
+    plain_batch_kernel<T, R><<<tiles, kThreads, 0, s>>>(ens, prefix, n_ens);
 }
 
 }  // namespace
 
+int plain_block(int n, bool fp64, bool alone, int n_cu)
+{
+    // cost of one tile relative to R = 4 (fp32) or R = 2 (fp64), per pair slot:
+    // smaller blocks re-read more LDS per multiply-add
+    static const double cost32[3] = {3.0, 1.6, 1.0};  // R = 1, 2, 4
+    static const double cost64[2] = {1.8, 1.0};       // R = 1, 2
+    double best = 0.0;
+    int best_r = 1;
+    for (int r = 1; r <= (fp64 ? 2 : 4); r *= 2) {
+        const double nt = plain_tiles_per_side(n, r);
+        const double tiles = nt * (nt + 1) / 2;
+        // a lone launch runs ceil(tiles / slots) rounds of 2 work-groups per CU
+        const double rounds = alone ? std::ceil(tiles / (2.0 * n_cu)) * (2.0 * n_cu) : tiles;
+        const double c = (fp64 ? cost64[r == 1 ? 0 : 1] : cost32[r == 1 ? 0 : r == 2 ? 1 : 2]) *
+                         (16.0 * r) * (16.0 * r) * rounds;
+        if (r == 1 || c < best) {
+            best = c;
+            best_r = r;
+        }
+    }
+    return best_r;
+}
+
 void launch_plain_avsa(const double* xyz, int n, int L, bool fp64, void* out, bool out_f64,
-                       hipStream_t s)
+                       int n_cu, hipStream_t s)
 {
     if (n < 2) return;
-    if (fp64)
-        launch<double, 2>(xyz, n, L, out, out_f64, s);
-    else
-        launch<float, 4>(xyz, n, L, out, out_f64, s);
+    const int r = plain_block(n, fp64, true, n_cu);
+    if (fp64) {
+        if (r == 2)
+            launch<double, 2>(xyz, n, L, out, out_f64, s);
+        else
+            launch<double, 1>(xyz, n, L, out, out_f64, s);
+    } else {
+        if (r == 4)
+            launch<float, 4>(xyz, n, L, out, out_f64, s);
+        else if (r == 2)
+            launch<float, 2>(xyz, n, L, out, out_f64, s);
+        else
+            launch<float, 1>(xyz, n, L, out, out_f64, s);
+    }
+}
+
+void launch_plain_batch(const PlainEns* ens, const int* prefix, int n_ens, int tiles, int r, bool fp64,
+                        hipStream_t s)
+{
+    if (n_ens == 0 || tiles == 0) return;
+    if (fp64) {
+        if (r == 2)
+            launch_batch<double, 2>(ens, prefix, n_ens, tiles, s);
+        else
+            launch_batch<double, 1>(ens, prefix, n_ens, tiles, s);
+    } else {
+        if (r == 4)
+            launch_batch<float, 4>(ens, prefix, n_ens, tiles, s);
+        else if (r == 2)
+            launch_batch<float, 2>(ens, prefix, n_ens, tiles, s);
+        else
+            launch_batch<float, 1>(ens, prefix, n_ens, tiles, s);
+    }
 }
 
 }  // namespace bs

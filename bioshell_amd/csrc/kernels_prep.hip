@@ -6,31 +6,15 @@
 namespace bs {
 namespace {
 
-// Reference: centroid and fit onto conformation 0 for each of the S samples
-// (wave each), slices of the rotated sum, the mean (thread per coordinate,
-// one wave per work-group to spread the few coordinates over more CUs),
-// then R0 (one block).
+// Reference: the fits of the S samples onto conformation 0 and their
+// rotated sums, one slice of samples per work-group (fit0_slice),
+// the mean (thread per coordinate, one wave per work-group to spread the few
+// coordinates over more CUs), then R0 (one block).
 __global__ __launch_bounds__(256) void fit0_kernel(const double* __restrict__ xyz, int n, int S,
-                                                   int L, double* __restrict__ rot,
-                                                   double* __restrict__ cent)
+                                                   int L, double* __restrict__ partial)
 {
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= S) return;
-    const int i = ref_sample_index(t, n, S);
-    fit0_one(xyz + (int64_t)i * L * 3, xyz, L, rot + (int64_t)t * 9, cent + (int64_t)t * 3,
-             threadIdx.x & 63);
-}
-
-__global__ __launch_bounds__(256) void mean_partial_kernel(const double* __restrict__ xyz,
-                                                           const double* __restrict__ cent,
-                                                           const double* __restrict__ rot, int n,
-                                                           int S, int L,
-                                                           double* __restrict__ partial)
-{
-    __shared__ double sR[kMeanSliceConfs][9];
-    __shared__ double sC[kMeanSliceConfs][3];
-    __shared__ int64_t sX[kMeanSliceConfs];
-    mean_partial_block(xyz, cent, rot, n, S, L, blockIdx.x, partial, sR, sC, sX);
+    __shared__ Fit0Lds lds;
+    fit0_slice(xyz, n, S, L, blockIdx.x, partial, lds);
 }
 
 __global__ __launch_bounds__(64) void mean_final_kernel(const double* __restrict__ partial,
@@ -57,14 +41,10 @@ __global__ __launch_bounds__(256) void fit_pack_kernel(const double* __restrict_
                                                        uint4* __restrict__ panels,
                                                        double* __restrict__ consts, int ld)
 {
-    __shared__ uint4 img[4][kPackWordsU4];
-    const int wv = threadIdx.x >> 6;
-    const int blk = xcd_contiguous_block((n_pad + 3) / 4);
+    __shared__ FitPackLds lds;
+    const int blk = xcd_contiguous_block(fit_groups(n_pad));
     if (blk < 0) return;
-    const int i = blk * 4 + wv;  // the logical blocks cover n_pad exactly
-    if (i >= n_pad) return;
-    fit_pack_one<MODE>(xyz, ref, n, L, n_kblk, i, rot, cent, panels, consts, ld, img[wv],
-                       threadIdx.x & 63);
+    fit_pack_block<MODE>(xyz, ref, n, L, n_kblk, blk * kFitConfs, rot, cent, panels, consts, ld, lds);
 }
 
 // ------------------------------------------------------ single superposition
@@ -112,13 +92,11 @@ __global__ __launch_bounds__(64) void superpose_kernel(const double* __restrict_
 
 }  // namespace
 
-void launch_reference(const double* xyz, int n, int L, double* rot, double* cent, double* partial,
-                      double* ref, hipStream_t s)
+void launch_reference(const double* xyz, int n, int L, double* partial, double* ref, hipStream_t s)
 {
     const int S = ref_sample_count(n);
-    const int slices = (S + kMeanSliceConfs - 1) / kMeanSliceConfs;
-    fit0_kernel<<<(S + 3) / 4, 256, 0, s>>>(xyz, n, S, L, rot, cent);
-    mean_partial_kernel<<<slices, 256, 0, s>>>(xyz, cent, rot, n, S, L, partial);
+    const int slices = ref_slices(S);
+    fit0_kernel<<<slices, 256, 0, s>>>(xyz, n, S, L, partial);
     mean_final_kernel<<<(3 * L + 63) / 64, 64, 0, s>>>(partial, slices, S, L, ref);
     ref_gram_kernel<<<1, 256, 0, s>>>(ref, L);
 }
@@ -127,7 +105,7 @@ void launch_fit_pack(const double* xyz, const double* ref, int n, int n_pad, int
                      GramMode mode, double* rot, double* cent, void* panels, double* consts, int ld,
                      hipStream_t s)
 {
-    const int blocks = xcd_contiguous_grid((n_pad + 3) / 4);
+    const int blocks = xcd_contiguous_grid(fit_groups(n_pad));
     uint4* p = static_cast<uint4*>(panels);
     if (mode == kGramBF16x3)
         fit_pack_kernel<kGramBF16x3><<<blocks, 256, 0, s>>>(xyz, ref, n, n_pad, L, n_kblk, rot,

@@ -136,7 +136,7 @@ bs_status bs_crmsd_all_vs_all_device_noalloc(bs_context* ctx, const double* d_xy
         const hipStream_t s = ctx->stream;
         const int n = (int)n_conf, L = (int)n_atoms;
         if (!is_gram(method)) {  // the per-pair kernel needs no workspace
-            launch_plain_avsa(d_xyz, n, L, method == BS_FP64_PLAIN, d_out_upper, out_f64 != 0, s);
+            launch_plain_avsa(d_xyz, n, L, method == BS_FP64_PLAIN, d_out_upper, out_f64 != 0, ctx->n_cu, s);
             BS_HIP(hipGetLastError());
             return;
         }

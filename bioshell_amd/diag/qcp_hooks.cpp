@@ -15,6 +15,9 @@ double bsi_qcp_rmsd_f64(const double* M, double Ga, double Gb, int n)
 // preparation kernels compute it.
 void bsi_rotation_fast(const double* M, double* R) { bs::rotation_fast(M, R); }
 
+// The same with Newton started from an upper bound of lambda_max (E0).
+void bsi_rotation_fast_from(const double* M, double upper, double* R) { bs::rotation_fast(M, R, upper); }
+
 float bsi_qcp_rmsd_f32(const float* M, float Ga, float Gb, int n)
 {
     return bs::qcp_rmsd<float>(M, Ga, Gb, n);

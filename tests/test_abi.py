@@ -261,6 +261,24 @@ def test_host_build_of_fast_rotation():
             R, ac, bc = rot_of(a, b)
             fit = np.sqrt(((bc @ R.T - ac) ** 2).sum() / L)
             assert fit == pytest.approx(O.crmsd(a, b), abs=1e-9)
+    # Newton started from E0 = (G_a + G_b) / 2 (the preparation kernels' bound),
+    # and from E0 rounded 1e-13 below lambda_max: the same optimal rotation
+    fn_from = _hook("bsi_rotation_fast_from", None, [dp, ctypes.c_double, dp])
+    for L in (3, 10, 76, 300):
+        for _ in range(10):
+            a = rng.normal(size=(L, 3)) * 5 + rng.normal(size=3) * 100
+            b = a @ rand_rot(rng).T + rng.normal(size=(L, 3)) * rng.uniform(0.01, 3.0)
+            R0, ac, bc = rot_of(a, b)
+            M = np.ascontiguousarray(bc.T @ ac)
+            e0 = 0.5 * ((ac ** 2).sum() + (bc ** 2).sum())
+            lam = e0 - 0.5 * L * O.crmsd(a, b) ** 2
+            for upper in (e0, lam * (1 - 1e-13)):
+                R = np.empty(9)
+                fn_from(M.ravel().ctypes.data_as(dp), upper, R.ctypes.data_as(dp))
+                R = R.reshape(3, 3)
+                assert np.abs(R - R0).max() < 1e-9
+                fit = np.sqrt(((bc @ R.T - ac) ** 2).sum() / L)
+                assert fit == pytest.approx(O.crmsd(a, b), abs=1e-9)
     # degenerate: M = 0 (one atom), collinear pairs, coplanar sets, identical sets
     for a, b in [(np.zeros((1, 3)), np.ones((1, 3))),
                  (np.array([[0.0, 0, 0], [3.8, 0, 0]]), np.array([[1.0, 2, 3], [1.0, 5.8, 3]])),

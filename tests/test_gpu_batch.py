@@ -95,6 +95,24 @@ def test_batch_large_ensemble_sampled_reference(gpu_ctx):
         assert np.abs(g.astype(np.float64) - O.all_vs_all(x, nthreads=8)).max() <= TOL_GRAM
 
 
+def test_batch_mixed_fit_shapes(gpu_ctx):
+    """Ensembles whose chains take every preparation shape (16 .. 256 threads
+    per conformation, and the streamed form beyond 2048 atoms) in one batch:
+    each triangle equals the single-ensemble call bit for bit and meets the
+    oracle gate; the per-pair kernel's batched launch (tile sides by ensemble
+    size) meets the FP64 gate."""
+    xs = make_batch(800, [(70, 30), (40, 128), (33, 200), (50, 400), (20, 900), (12, 1500),
+                          (9, 2100), (17, 129), (5, 2049)])
+    got = B.crmsd_batch(xs, B.Method.BF16X3_GRAM, ctx=gpu_ctx)
+    plain = B.crmsd_batch(xs, B.Method.FP64_PLAIN, ctx=gpu_ctx)
+    for x, g, p in zip(xs, got, plain):
+        single = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx).upper
+        assert np.array_equal(g, single), f"shape {x.shape}"
+        want = O.all_vs_all(x, nthreads=8)
+        assert np.abs(g.astype(np.float64) - want).max() <= TOL_GRAM, f"shape {x.shape}"
+        assert np.abs(p.astype(np.float64) - want).max() <= TOL_FP64 + 1e-6 * max(1.0, want.max())
+
+
 def test_batch_graph_replay_and_rebuild(gpu_ctx, batch):
     """The captured graph is replayed for a repeated shape and rebuilt for a new
     one; every replay gives the eager-equivalent (first) result."""

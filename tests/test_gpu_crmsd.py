@@ -119,6 +119,45 @@ def test_random_shapes_and_scales(gpu_ctx, seed):
         assert err.max() <= max(tol, TOL[method]), (method, n, L)
 
 
+@pytest.mark.parametrize("L", [128, 129, 256, 257, 512, 513, 1024, 1025, 2048, 2049, 2600])
+def test_chain_lengths_across_fit_shapes(gpu_ctx, L):
+    """The preparation fits each conformation with 16 .. 256 threads holding up
+    to 8 atoms each (lane count doubling at 128, 256, 512, 1024 atoms) and
+    streams chains beyond 2048 atoms; every shape and both Gram modes against
+    the oracle, n = 45 (padding rows in the last work-group)."""
+    x = B.generate_ensemble(4000 + L, 45, L, sigma=(0.5, 3.0), near_fraction=0.1)
+    want = O.all_vs_all(x, nthreads=8)
+    for method in (B.Method.BF16X3_GRAM, B.Method.FP32_GRAM, B.Method.FP64_PLAIN):
+        err = np.abs(run(gpu_ctx, x, method) - want)
+        assert err.max() <= TOL[method], (method.name, L, err.max())
+
+
+def test_results_independent_of_earlier_calls(gpu_ctx):
+    """The same ensemble gives the same triangle bit for bit whatever ran on the
+    device before (other shapes, streamed blocks, batches): no kernel reads
+    memory or LDS it has not written."""
+    x = B.generate_ensemble(4200, 257, 300, sigma=(0.5, 4.0), near_fraction=0.05)
+    for m in (B.Method.BF16X3_GRAM, B.Method.FP32_GRAM):
+        first = B.crmsd_all_vs_all(x, m, ctx=gpu_ctx).upper
+        B.crmsd_all_vs_all_streamed(x, m, tile=64, ctx=gpu_ctx)
+        B.crmsd_batch([B.generate_ensemble(4201, 90, 700), B.generate_ensemble(4202, 33, 2100)], m,
+                      ctx=gpu_ctx)
+        B.crmsd_all_vs_all(B.generate_ensemble(4203, 500, 40), m, ctx=gpu_ctx)
+        again = B.crmsd_all_vs_all(x, m, ctx=gpu_ctx).upper
+        assert np.array_equal(first, again), m.name
+
+
+@pytest.mark.parametrize("n", [17, 40, 100, 250, 700])
+def test_plain_tile_blocks(gpu_ctx, n):
+    """The per-pair kernel picks a tile side of 16, 32 or 64 conformations by
+    ensemble size; every choice in both precisions against the oracle."""
+    x = B.generate_ensemble(4100 + n, n, 57, sigma=(0.5, 3.0))
+    want = O.all_vs_all(x, nthreads=8)
+    for method in (B.Method.FP64_PLAIN, B.Method.FP32_PLAIN):
+        err = np.abs(run(gpu_ctx, x, method) - want)
+        assert err.max() <= TOL[method], (method.name, n, err.max())
+
+
 def test_single_conformation_and_errors(gpu_ctx):
     x = B.generate_ensemble(1, 1, 10)
     assert B.crmsd_all_vs_all(x, ctx=gpu_ctx).upper.size == 0
