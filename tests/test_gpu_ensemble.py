@@ -40,7 +40,8 @@ def check_fit(fit, x, ref, unique=True):
             assert abs(resid2 - r0 ** 2) <= 1e-11 * max(1.0, ms) + 1e-16 * scale ** 2
 
 
-@pytest.mark.parametrize("n,L,seed", [(5, 3, 1), (17, 5, 2), (200, 76, 3), (64, 300, 4), (9, 1100, 5)])
+@pytest.mark.parametrize("n,L,seed", [(5, 3, 1), (17, 5, 2), (200, 76, 3), (64, 300, 4), (9, 1100, 5),
+                                      (33, 129, 6), (21, 256, 7), (10, 600, 8), (5, 2048, 9), (3, 2600, 10)])
 def test_matches_oracle(gpu_ctx, n, L, seed):
     x = B.generate_ensemble(seed, n, L, sigma=(0.5, 3.0))
     fit = B.superpose_ensemble(x, ref_index=n // 2, ctx=gpu_ctx)
