@@ -70,6 +70,10 @@ struct bs_server {
     std::condition_variable cv_work, cv_done;
     std::deque<ReqPtr> queue;
     std::unordered_map<uint64_t, ReqPtr> tickets;
+    // query copies of finished requests, reused by later submits: a fresh
+    // buffer of a few hundred KiB would be new pages, faulted in on the copy
+    std::vector<std::vector<double>> q_pool;
+    static constexpr size_t kQueryPool = 32;
     uint64_t next_ticket = 1;
     bool stop = false;
     std::thread worker;
@@ -78,6 +82,8 @@ struct bs_server {
     // A request reached its final status: a released ticket's record goes.
     void ended_locked(const ReqPtr& r)
     {
+        if (q_pool.size() < kQueryPool && r->q.capacity() > 0) q_pool.push_back(std::move(r->q));
+        r->q = std::vector<double>();
         if (r->released) tickets.erase(r->ticket);
     }
     void fail_slot_locked(Slot& s, bs_status code, const char* msg);
@@ -421,6 +427,13 @@ bs_status bs_server_submit(bs_server* srv, size_t lib_index, const double* q_xyz
         auto r = std::make_shared<Request>();
         r->t_submit = Clock::now();
         r->lib = lib_index;
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            if (!srv->q_pool.empty()) {
+                r->q = std::move(srv->q_pool.back());
+                srv->q_pool.pop_back();
+            }
+        }
         r->q.assign(q_xyz, q_xyz + n_q * (size_t)lib.L * 3);
         r->n_q = (int)n_q;
         r->k = (int)k;
