@@ -6,7 +6,8 @@
 //            c_i = (sum x_i)/n (substructures.rs:27-42) and the rotation U_i
 //            that superposes x_i - c_i onto x_0 - c_0
 //   mean     r = mean over the samples of U_i (x_i - c_i): one
-//            Procrustes-mean step, r centred; then R0 = sum_k r_k r_k^T
+//            Procrustes-mean step, r centred (partial sums of 16 samples in
+//            the fit0 pass, then a fixed-order sum); then R0 = sum_k r_k r_k^T
 //   fit+pack U_i onto r from M = sum_k x_k r_k^T (with r centred, raw
 //            coordinates give the centred covariance), then
 //            e_i = U_i (x_i - c_i) - r split into MFMA operand panels, and
@@ -16,11 +17,11 @@
 //
 // Rotations come from Horn's quaternion (rotation_fast): the largest root of
 // the key matrix's characteristic quartic by Newton from above, the
-// eigenvector from the adjugate of K - lambda I; every lane of the wave
-// computes it from the wave-reduced covariance, so no extra pass is needed.
-// The first two passes read the S samples (24 bytes per atom each), the fit
-// and pack reads all n conformations and writes 12 bytes per atom; all of it
-// is O(n L) and runs once per ensemble, none of it per pair.
+// eigenvector from the adjugate of K - lambda I, one thread per conformation
+// from the work-group's reduced sums.  The sample fits read the S samples
+// twice (the fit, then the rotated sum), the fit and pack reads all n
+// conformations twice and writes 12 bytes per atom; all of it is O(n L) and
+// runs once per ensemble, none of it per pair.
 #pragma once
 
 #include "kernels.hpp"
