@@ -50,7 +50,9 @@ __host__ __device__ inline int tile_row_first(int r, int T) { return r * T - r *
 // Block b covers tile base + b = (tr, tc), tr <= tc: rows 64 tr.., columns
 // 64 tc.. of the packed triangle.  Wave w takes rows 64 tr + 16 w .. + 15; lane l holds
 // column 64 tc + l, so every row's 64 values are one 256-byte read, and the
-// wave issues its 16 reads before using any.  The tile's results go to
+// wave issues its 16 reads before using any.  Columns fold in each lane's
+// registers; rows fold through an LDS copy of the tile's keys, four threads
+// per row.  The tile's results go to
 // part[256 b ..]: row minima, row maxima, column minima, column maxima (64
 // keys each; a row or column with no entry here holds the identity).  base
 // is the first tile of the chunk of tile rows this launch covers.
@@ -63,6 +65,7 @@ __global__ __launch_bounds__(256) void row_stats_tile_kernel(const float* __rest
     while (tr + 1 < T && tile_row_first(tr + 1, T) <= b) ++tr;
     const int tc = tr + (b - tile_row_first(tr, T));
     __shared__ unsigned cmin[4][64], cmax[4][64];
+    __shared__ unsigned tmin[64][65], tmax[64][65];  // the tile's keys, [row][column]
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int j = tc * 64 + lane;
     const int i0 = tr * 64 + w * 16;
@@ -73,6 +76,7 @@ __global__ __launch_bounds__(256) void row_stats_tile_kernel(const float* __rest
         const int i = i0 + rr;
         d[rr] = (i < n && j > i && j < n) ? up[tri(n, i, j)] : 0.0f;
     }
+    // columns: each lane folds its 16 rows; rows: through LDS below
     unsigned clo = 0xFFFFFFFFu, chi = 0u;
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
@@ -82,15 +86,30 @@ __global__ __launch_bounds__(256) void row_stats_tile_kernel(const float* __rest
         const unsigned klo = ok ? k : 0xFFFFFFFFu, khi = ok ? k : 0u;
         clo = min(clo, klo);
         chi = max(chi, khi);
-        const unsigned rlo = wave_min_u32(klo), rhi = wave_max_u32(khi);
-        if (lane == 0) {
-            out[w * 16 + rr] = rlo;
-            out[64 + w * 16 + rr] = rhi;
-        }
+        tmin[w * 16 + rr][lane] = klo;
+        tmax[w * 16 + rr][lane] = khi;
     }
     cmin[w][lane] = clo;
     cmax[w][lane] = chi;
     __syncthreads();
+    {
+        // row r of the tile: four threads of 16 columns each, then two shuffles
+        const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+        unsigned lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            lo = min(lo, tmin[r][q * 16 + c]);
+            hi = max(hi, tmax[r][q * 16 + c]);
+        }
+        lo = min(lo, (unsigned)__shfl_xor((int)lo, 1));
+        hi = max(hi, (unsigned)__shfl_xor((int)hi, 1));
+        lo = min(lo, (unsigned)__shfl_xor((int)lo, 2));
+        hi = max(hi, (unsigned)__shfl_xor((int)hi, 2));
+        if (q == 0) {
+            out[r] = lo;
+            out[64 + r] = hi;
+        }
+    }
     if (w == 0) {
         out[128 + lane] = min(min(cmin[0][lane], cmin[1][lane]), min(cmin[2][lane], cmin[3][lane]));
         out[192 + lane] = max(max(cmax[0][lane], cmax[1][lane]), max(cmax[2][lane], cmax[3][lane]));

@@ -44,25 +44,13 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Minimum / maximum of an unsigned value over the 64 lanes (all active),
-// returned in every lane: the row-shift and row-broadcast pattern of
-// wave_sum, with lanes outside a row or a mask reading the identity.
+// One DPP step on an unsigned value: the row-shift and row-broadcast
+// pattern of wave_sum, with lanes outside a row or a mask reading the
+// identity.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ unsigned dpp_u32(unsigned x, unsigned identity)
 {
     return (unsigned)__builtin_amdgcn_update_dpp((int)identity, (int)x, CTRL, ROW_MASK, 0xF, false);
-}
-
-__device__ inline unsigned wave_min_u32(unsigned v)
-{
-    constexpr unsigned I = 0xFFFFFFFFu;
-    v = min(v, dpp_u32<0x111, 0xF>(v, I));
-    v = min(v, dpp_u32<0x112, 0xF>(v, I));
-    v = min(v, dpp_u32<0x114, 0xF>(v, I));
-    v = min(v, dpp_u32<0x118, 0xF>(v, I));
-    v = min(v, dpp_u32<0x142, 0xA>(v, I));
-    v = min(v, dpp_u32<0x143, 0xC>(v, I));
-    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // Minimum of a 64-bit unsigned value over the 64 lanes (all active), in
@@ -87,17 +75,6 @@ __device__ inline uint64_t wave_min_u64(uint64_t v)
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
     return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ inline unsigned wave_max_u32(unsigned v)
-{
-    v = max(v, dpp_u32<0x111, 0xF>(v, 0u));
-    v = max(v, dpp_u32<0x112, 0xF>(v, 0u));
-    v = max(v, dpp_u32<0x114, 0xF>(v, 0u));
-    v = max(v, dpp_u32<0x118, 0xF>(v, 0u));
-    v = max(v, dpp_u32<0x142, 0xA>(v, 0u));
-    v = max(v, dpp_u32<0x143, 0xC>(v, 0u));
-    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 }  // namespace bs
