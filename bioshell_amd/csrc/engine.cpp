@@ -173,18 +173,21 @@ PrepView view_of(const Prepared& P, int first, int count)
 }
 
 void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, const uint32_t* order,
-                   int n_tiles, uint32_t* flags, int list_cap, int max_wg, hipStream_t s, RunRecord* rec)
+                   int n_tiles, uint32_t* flags, int list_cap, bool clear_flags, int max_wg, hipStream_t s,
+                   RunRecord* rec)
 {
-    // Flagged outputs are marked in a bitmap indexed by output position (one
-    // bit per pair: 6.25 MB for 50M pairs), so there is no capacity limit and
-    // no host round trip; after the Gram kernel the set bits are queued (the
-    // word after the bitmap counts them) and re-evaluated one wave each.
+    // Flagged outputs go to a list of list_cap positions (counted in the
+    // control words at the region's start); any beyond it set bits of a
+    // bitmap indexed by output position (one bit per pair: 6.25 MB for 50M
+    // pairs), so there is no capacity limit and no host round trip.  One
+    // kernel then re-evaluates them in FP64 and leaves the region clean.
     const uint64_t n_out =
         run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * (uint64_t)run.out_ld;
     const uint64_t n_words = flag_words(n_out);
-    uint32_t* count = flags + n_words;
-    int64_t* list = reinterpret_cast<int64_t*>(flags + n_words + 4);
-    launch_zero_words(flags, (int64_t)n_words + 4, s);
+    uint32_t* ctl = flags;  // count, report, done, spare
+    int64_t* list = reinterpret_cast<int64_t*>(flags + 4);
+    uint32_t* bits = reinterpret_cast<uint32_t*>(list + list_cap);
+    if (clear_flags) launch_zero_words(flags, (int64_t)(flag_region_bytes(n_out, list_cap) / 4), s);
 
     GramArgs g{};
     g.panels_a = A.panels;
@@ -203,7 +206,10 @@ void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, con
     g.out_ld = run.out_ld;
     g.out = run.d_out;
     g.out_f64 = run.out_f64 ? 1 : 0;
-    g.flag_bits = flags;
+    g.flag_bits = bits;
+    g.flag_list = list;
+    g.flag_count = ctl;
+    g.flag_cap = list_cap;
     const GramThresholds th = gram_thresholds(A.mode, A.L, A.n_kblk);
     g.kappa_dup = th.kappa_dup;
     g.u_rep = th.u_rep;
@@ -211,17 +217,17 @@ void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, con
     g.tau = th.tau;
     g.rep_g_max = th.rep_g_max;
     g.max_wg = max_wg > 0 ? max_wg : 512;
+    (void)n_words;
 
-    if (rec) rec->mark({kBPrep1, kBMain0}, s);  // the bitmap clear counts as preparation
+    if (rec) rec->mark({kBPrep1, kBMain0}, s);
     launch_gram(g, A.mode, s);
     if (rec) rec->mark({kBMain1, kBFallback0}, s);
     BS_HIP(hipGetLastError());
-    launch_fallback(A.xyz, A.cent, B.xyz, B.cent, A.L, flags, (int64_t)n_out, g.triangle, A.n, run.out_ld,
-                    run.d_out, g.out_f64, count, list, list_cap, s);
+    launch_flag_resolve(A.xyz, A.cent, B.xyz, B.cent, A.L, ctl, list, list_cap, bits, (int64_t)n_out,
+                        g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, s);
     BS_HIP(hipGetLastError());
-    if (rec && rec->h_count)
-        BS_HIP(hipMemcpyAsync(rec->h_count, count, 4, hipMemcpyDeviceToHost, s));
-    if (run.h_count) BS_HIP(hipMemcpyAsync(run.h_count, count, 4, hipMemcpyDeviceToHost, s));
+    if (rec && rec->h_count) BS_HIP(hipMemcpyAsync(rec->h_count, ctl + 1, 4, hipMemcpyDeviceToHost, s));
+    if (run.h_count) BS_HIP(hipMemcpyAsync(run.h_count, ctl + 1, 4, hipMemcpyDeviceToHost, s));
     if (rec) {
         if (run.ends_total)
             rec->mark({kBFallback1, kBTotal1}, s);
@@ -241,9 +247,14 @@ void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScra
     const uint32_t* order = sc.tile_order(tiles_r, tiles_c, run.triangle, s, &n_tiles);
     const uint64_t n_out =
         run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * (uint64_t)run.out_ld;
+    // every launch leaves the region clean: only fresh memory needs zeroing
+    const void* before = sc.flags.get();
+    const size_t before_bytes = sc.flags.size();
     uint32_t* flags = static_cast<uint32_t*>(
         sc.flags.ensure(flag_region_bytes(n_out, kFlagListCap), "flag bitmap and queue"));
-    run_gram_with(A, B, run, order, n_tiles, flags, kFlagListCap, sc.max_wg, s, rec);
+    if (flags != before || sc.flags.size() != before_bytes)
+        launch_zero_words(flags, (int64_t)(sc.flags.size() / 16 * 4), s);
+    run_gram_with(A, B, run, order, n_tiles, flags, kFlagListCap, false, sc.max_wg, s, rec);
 }
 
 void GramScratch::reset()

@@ -166,16 +166,22 @@ struct RunRecord {
 // the flagged-pair count is copied to rec->h_count when rec is given.
 // Words of the flag bitmap of n_out outputs (a whole number of 16-byte units).
 inline uint64_t flag_words(uint64_t n_out) { return ((n_out + 31) / 32 + 3) / 4 * 4; }
-// Bytes of the flag region: the bitmap, a 16-byte count word, list_cap queued entries.
+// Bytes of the flag region: four control words (count, report, done, spare),
+// list_cap queued entries, then the bitmap.  The list sits at a fixed offset
+// and the bitmap grows at the end, so a region reused for a larger launch
+// never finds list entries where its bitmap is.
 inline size_t flag_region_bytes(uint64_t n_out, int list_cap)
 {
-    return (size_t)(flag_words(n_out) + 4) * 4 + (size_t)list_cap * 8;
+    return 16 + (size_t)list_cap * 8 + (size_t)flag_words(n_out) * 4;
 }
 // The Gram launch sequence with every buffer given: `order` (n_tiles tile
 // codes on the device), `flags` (flag_region_bytes(n_out, list_cap)); no
 // allocation, no host copy except the optional counts of `run` / `rec`.
+// Each launch leaves the region's control words zero and its bitmap clear;
+// `clear_flags` zeroes the region first, for memory that may hold anything.
 void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, const uint32_t* order,
-                   int n_tiles, uint32_t* flags, int list_cap, int max_wg, hipStream_t s, RunRecord* rec);
+                   int n_tiles, uint32_t* flags, int list_cap, bool clear_flags, int max_wg, hipStream_t s,
+                   RunRecord* rec);
 void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScratch& sc,
               hipStream_t s, RunRecord* rec);
 

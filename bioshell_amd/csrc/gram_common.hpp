@@ -115,10 +115,19 @@ __device__ inline double fallback_pair(const double* __restrict__ x, const doubl
     return qcp_rmsd<double>(m9, ga, gb, L);
 }
 
-// Marks output o for FP64 re-evaluation.
+// Marks output o for FP64 re-evaluation: queued in the list while it has
+// room, else a bit in the bitmap.
 __device__ __forceinline__ void flag_output(const GramArgs& a, bool flag, int64_t o)
 {
-    if (flag) atomicOr(a.flag_bits + (o >> 5), 1u << (unsigned)(o & 31));
+    if (!flag) return;
+    if (a.flag_list) {
+        const uint32_t pos = atomicAdd(a.flag_count, 1u);
+        if (pos < (uint32_t)a.flag_cap) {
+            a.flag_list[pos] = o;
+            return;
+        }
+    }
+    atomicOr(a.flag_bits + (o >> 5), 1u << (unsigned)(o & 31));
 }
 
 }  // namespace bs

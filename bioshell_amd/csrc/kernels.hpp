@@ -100,6 +100,12 @@ struct GramArgs {
     void* out;
     int out_f64;
     uint32_t* flag_bits;    // bit o set: output o is re-evaluated in FP64
+    // when flag_list is set, flagged outputs are queued there first
+    // (*flag_count counts every flagged output; the bitmap takes those
+    // beyond flag_cap)
+    int64_t* flag_list;
+    uint32_t* flag_count;
+    int flag_cap;
     // flagging thresholds
     double kappa_dup;       // H < kappa_dup * u_rep * sqrt(g_i g_j)
     double u_rep;
@@ -117,15 +123,16 @@ int build_tile_order(int tiles_r, int tiles_c, bool triangle, int super, uint32_
 void launch_gram(const GramArgs& a, GramMode mode, hipStream_t s);
 void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
-// FP64 re-evaluation of the outputs whose bit is set in `bits` (n_out bits),
-// from raw coordinates and centroids of rows (a) and columns (b): the set bits
-// are queued into `list` (cap entries; *count, zeroed by the caller, receives
-// the number flagged), one wave re-evaluates each entry, and bits that did not
-// fit in the list are re-evaluated by a scan of the bitmap.
-void launch_fallback(const double* xyz_a, const double* cent_a, const double* xyz_b,
-                     const double* cent_b, int L, uint32_t* bits, int64_t n_out, int triangle,
-                     int n, int64_t out_ld, void* out, int out_f64, uint32_t* count, int64_t* list,
-                     int cap, hipStream_t s);
+// FP64 re-evaluation of the outputs the Gram kernel flagged, from raw
+// coordinates and centroids of rows (a) and columns (b): one wave per entry of
+// `list` (min(count, cap) entries), then, only when count > cap, a scan of the
+// bitmap `bits` (n_out bits) that clears every word it reads.  ctl holds
+// count, report, done: the last work-group to finish copies count to report
+// and zeroes count and done, so the region is ready for the next launch.
+void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double* xyz_b,
+                         const double* cent_b, int L, uint32_t* ctl, const int64_t* list, int cap,
+                         uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld, void* out,
+                         int out_f64, hipStream_t s);
 // Queues the set bits of bits[0 .. n_words) as bit indices (word * 32 + bit)
 // into list (cap entries), clearing the queued ones; *count += set bits.
 void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,

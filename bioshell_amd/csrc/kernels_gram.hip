@@ -58,7 +58,8 @@ __global__ void zero_words_kernel(uint4* p, int64_t n4)
 // Queues the set bits of the flag bitmap: one thread per word; a wave scan of
 // the words' bit counts and one atomic per wave give each word its slots.
 // Queued bits are cleared; bits that do not fit in the list (cap entries)
-// stay set for fallback_kernel.  *count receives the number of set bits.
+// stay set for a bitmap scan.  *count receives the number of set bits.  (The
+// batched pipeline's flag path; a single Gram launch queues directly.)
 __global__ __launch_bounds__(256) void flag_compact_kernel(uint32_t* __restrict__ bits,
                                                            int64_t n_words,
                                                            int64_t* __restrict__ list, int cap,
@@ -111,74 +112,75 @@ __device__ inline void decode_output(int64_t o, int triangle, int n, int64_t out
     }
 }
 
-// FP64 re-evaluation of the queued outputs: one wave per entry, the 64 lanes
-// splitting the atoms.
-__global__ __launch_bounds__(256) void flag_eval_kernel(const int64_t* __restrict__ list,
-                                                        const uint32_t* __restrict__ count, int cap,
-                                                        const double* __restrict__ xa,
-                                                        const double* __restrict__ ca,
-                                                        const double* __restrict__ xb,
-                                                        const double* __restrict__ cb, int L,
-                                                        int triangle, int n, int64_t out_ld,
-                                                        void* __restrict__ out, int out_f64)
+// FP64 re-evaluation of one flagged output o (lane 0 stores the result).
+__device__ inline void resolve_output(int64_t o, const double* __restrict__ xa, const double* __restrict__ ca,
+                                      const double* __restrict__ xb, const double* __restrict__ cb, int L,
+                                      int triangle, int n, int64_t out_ld, void* __restrict__ out, int out_f64,
+                                      int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t m = min((int64_t)*count, (int64_t)cap);
-    for (int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); k < m; k += (int64_t)gridDim.x * 4) {
-        const int64_t o = list[k];
-        int i, j;
-        decode_output(o, triangle, n, out_ld, i, j);
-        const double r = fallback_pair(xa + (int64_t)i * L * 3, ca + 3 * (int64_t)i,
-                                       xb + (int64_t)j * L * 3, cb + 3 * (int64_t)j, L, lane);
-        if (lane == 0) {
-            if (out_f64)
-                static_cast<double*>(out)[o] = r;
-            else
-                static_cast<float*>(out)[o] = (float)r;
-        }
+    int i, j;
+    decode_output(o, triangle, n, out_ld, i, j);
+    const double r = fallback_pair(xa + (int64_t)i * L * 3, ca + 3 * (int64_t)i, xb + (int64_t)j * L * 3,
+                                   cb + 3 * (int64_t)j, L, lane);
+    if (lane == 0) {
+        if (out_f64)
+            static_cast<double*>(out)[o] = r;
+        else
+            static_cast<float*>(out)[o] = (float)r;
     }
 }
 
-// FP64 re-evaluation of flagged outputs left in the bitmap (those the list
-// could not hold): each wave takes 64 bitmap words (one
-// per lane) and processes their set bits one pair at a time, the 64 lanes
-// splitting the atoms; textbook QCP with the near-degenerate polish.
-__global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ xa,
-                                                       const double* __restrict__ ca,
-                                                       const double* __restrict__ xb,
-                                                       const double* __restrict__ cb, int L,
-                                                       const uint32_t* __restrict__ bits,
-                                                       int64_t n_out, int triangle, int n,
-                                                       int64_t out_ld, void* __restrict__ out,
-                                                       int out_f64)
+// The flagged outputs of one Gram launch: one wave per queued entry, then,
+// only when more were flagged than the list holds, a scan of the bitmap in
+// which each wave takes 64 words (one per lane), processes their set bits one
+// pair at a time (the 64 lanes splitting the atoms) and clears the words.
+// ctl = {count, report, done}: the last work-group to finish publishes the
+// count in report and zeroes count and done for the next launch.
+__global__ __launch_bounds__(256) void flag_resolve_kernel(const double* __restrict__ xa,
+                                                           const double* __restrict__ ca,
+                                                           const double* __restrict__ xb,
+                                                           const double* __restrict__ cb, int L,
+                                                           uint32_t* __restrict__ ctl,
+                                                           const int64_t* __restrict__ list, int cap,
+                                                           uint32_t* __restrict__ bits, int64_t n_out,
+                                                           int triangle, int n, int64_t out_ld,
+                                                           void* __restrict__ out, int out_f64)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t n_words = (n_out + 31) >> 5;
+    const uint32_t cnt = ctl[0];  // read before this group counts itself done
     const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t n_waves = (int64_t)gridDim.x * 4;
-    for (int64_t w0 = wave * 64; w0 < n_words; w0 += n_waves * 64) {
-        const int64_t wi = w0 + lane;
-        uint32_t word = wi < n_words ? bits[wi] : 0u;
-        while (true) {
-            const uint64_t m = __ballot(word != 0u);
-            if (!m) break;
-            const int src = __ffsll((unsigned long long)m) - 1;
-            const uint32_t wsrc = __shfl(word, src);
-            const int bit = __ffs(wsrc) - 1;
-            const int64_t o = (w0 + src) * 32 + bit;
-            if (lane == src) word &= word - 1u;  // clear the lowest set bit
-            if (o >= n_out) continue;
-            int i, j;
-            decode_output(o, triangle, n, out_ld, i, j);
-            const double r = fallback_pair(xa + (int64_t)i * L * 3, ca + 3 * (int64_t)i,
-                                           xb + (int64_t)j * L * 3, cb + 3 * (int64_t)j, L, lane);
-            if (lane == 0) {
-                if (out_f64)
-                    static_cast<double*>(out)[o] = r;
-                else
-                    static_cast<float*>(out)[o] = (float)r;
+    const int64_t m = min((int64_t)cnt, (int64_t)cap);
+    for (int64_t k = wave; k < m; k += n_waves) {
+        const int64_t o = list[k];
+        if (o >= 0 && o < n_out) resolve_output(o, xa, ca, xb, cb, L, triangle, n, out_ld, out, out_f64, lane);
+    }
+    if ((int64_t)cnt > (int64_t)cap) {  // uniform: the bitmap holds the rest
+        const int64_t n_words = (n_out + 31) >> 5;
+        for (int64_t w0 = wave * 64; w0 < n_words; w0 += n_waves * 64) {
+            const int64_t wi = w0 + lane;
+            uint32_t word = wi < n_words ? bits[wi] : 0u;
+            if (word) bits[wi] = 0u;
+            while (true) {
+                const uint64_t msk = __ballot(word != 0u);
+                if (!msk) break;
+                const int src = __ffsll((unsigned long long)msk) - 1;
+                const uint32_t wsrc = __shfl(word, src);
+                const int bit = __ffs(wsrc) - 1;
+                const int64_t o = (w0 + src) * 32 + bit;
+                if (lane == src) word &= word - 1u;  // clear the lowest set bit
+                if (o >= n_out) continue;
+                resolve_output(o, xa, ca, xb, cb, L, triangle, n, out_ld, out, out_f64, lane);
             }
         }
+    }
+    // Every thread of the group has used its copy of the count (no fence:
+    // the reset only needs the reads done, and the next launch sees it).
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(&ctl[2], 1u) == gridDim.x - 1) {
+        ctl[1] = cnt;
+        ctl[0] = 0u;
+        ctl[2] = 0u;
     }
 }
 
@@ -220,18 +222,16 @@ void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s)
     zero_words_kernel<<<(int)blocks, 256, 0, s>>>(reinterpret_cast<uint4*>(p), n4);
 }
 
-void launch_fallback(const double* xyz_a, const double* cent_a, const double* xyz_b,
-                     const double* cent_b, int L, uint32_t* bits, int64_t n_out, int triangle,
-                     int n, int64_t out_ld, void* out, int out_f64, uint32_t* count, int64_t* list,
-                     int cap, hipStream_t s)
+void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double* xyz_b,
+                         const double* cent_b, int L, uint32_t* ctl, const int64_t* list, int cap,
+                         uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld, void* out,
+                         int out_f64, hipStream_t s)
 {
+    // enough waves for a full list or a bitmap scan; idle groups leave at once
     const int64_t n_words = (n_out + 31) / 32;
-    launch_flag_compact(bits, n_words, list, cap, count, s);
-    flag_eval_kernel<<<1024, 256, 0, s>>>(list, count, cap, xyz_a, cent_a, xyz_b, cent_b, L,
-                                          triangle, n, out_ld, out, out_f64);
-    const int64_t blocks = std::min<int64_t>(2048, std::max<int64_t>(1, (n_words + 255) / 256));
-    fallback_kernel<<<(int)blocks, 256, 0, s>>>(xyz_a, cent_a, xyz_b, cent_b, L, bits, n_out,
-                                                triangle, n, out_ld, out, out_f64);
+    const int64_t blocks = std::min<int64_t>(1024, std::max<int64_t>(1, (n_words + 255) / 256));
+    flag_resolve_kernel<<<(int)std::max<int64_t>(blocks, 256), 256, 0, s>>>(
+        xyz_a, cent_a, xyz_b, cent_b, L, ctl, list, cap, bits, n_out, triangle, n, out_ld, out, out_f64);
 }
 
 void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,

@@ -184,7 +184,8 @@ bs_status bs_crmsd_all_vs_all_device_noalloc(bs_context* ctx, const double* d_xy
         run.triangle = true;
         run.d_out = d_out_upper;
         run.out_f64 = out_f64 != 0;
-        run_gram_with(v, v, run, order, n_tiles, flags, kNoAllocListCap, ctx->gsc.max_wg, s, nullptr);
+        // the region's bytes held other data in earlier calls: zero it first
+        run_gram_with(v, v, run, order, n_tiles, flags, kNoAllocListCap, true, ctx->gsc.max_wg, s, nullptr);
     });
 }
 
