@@ -337,8 +337,7 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
         b.n_ref = P.n_ref[e];
         b.n_slices = P.slices[e];
         b.tile = P.tile[e];
-        b.acc_coef = P.th[e].acc_coef;
-        b.rep_g_max = P.th[e].rep_g_max;
+        b.lim = epilogue_limits(P.th[e], P.L[e]);
     }
     std::vector<unsigned char> blob;
     blob.reserve(4096);
@@ -372,9 +371,6 @@ void enqueue(bs_context* ctx, const Plan& P, const std::vector<const double*>& d
     a.ens_bits0 = bits;
     a.counters = bits + total_words;
     a.flag_list = reinterpret_cast<int64_t*>(bits + total_words + 16);
-    a.kappa_dup = P.th[0].kappa_dup;
-    a.u_rep = P.th[0].u_rep;
-    a.tau = P.th[0].tau;
     a.n_wg = std::max(1, std::min(2 * ctx->n_cu, (int)P.items.size()));
 
     // graph of this shape: the kernel arguments are all that it bakes in

@@ -75,6 +75,7 @@ void* DevBuf::ensure(size_t bytes, const char* what)
     const size_t want = bytes + bytes / 8;
     ptr_ = budget_->alloc(want, what);
     bytes_ = want;
+    ++gen_;
     return ptr_;
 }
 

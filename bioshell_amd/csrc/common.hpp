@@ -97,6 +97,9 @@ public:
     void reset();
     void* get() const { return ptr_; }
     size_t size() const { return bytes_; }
+    // Allocation count of this buffer (0: never allocated): a changed value
+    // means memory of unknown contents, even at the same address and size.
+    uint64_t generation() const { return gen_; }
     template <class T>
     T* as() const { return static_cast<T*>(ptr_); }
 
@@ -104,6 +107,7 @@ private:
     Budget* budget_ = nullptr;
     void* ptr_ = nullptr;
     size_t bytes_ = 0;
+    uint64_t gen_ = 0;
 };
 
 // Copies a list of (dst, src, bytes) spans with up to 16 threads.

@@ -110,6 +110,19 @@ GramThresholds gram_thresholds(GramMode mode, int L, int n_kblk)
     return t;
 }
 
+EpilogueLimits epilogue_limits(const GramThresholds& t, int L)
+{
+    EpilogueLimits e;
+    const double Ld = (double)L;
+    e.inv_L = 1.0 / Ld;
+    e.dup_coef = t.kappa_dup * t.u_rep;
+    e.acc_coef = t.acc_coef;
+    e.tau2L = t.tau * t.tau * Ld;
+    e.solve_tol2L = 0.0625 * e.tau2L;
+    e.rep_h_max = 0.5 * t.rep_g_max;  // h = g / 2
+    return e;
+}
+
 void RunRecord::create()
 {
     for (auto& e : ev) BS_HIP(hipEventCreate(&e));
@@ -210,12 +223,7 @@ void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, con
     g.flag_list = list;
     g.flag_count = ctl;
     g.flag_cap = list_cap;
-    const GramThresholds th = gram_thresholds(A.mode, A.L, A.n_kblk);
-    g.kappa_dup = th.kappa_dup;
-    g.u_rep = th.u_rep;
-    g.acc_coef = th.acc_coef;
-    g.tau = th.tau;
-    g.rep_g_max = th.rep_g_max;
+    g.lim = epilogue_limits(gram_thresholds(A.mode, A.L, A.n_kblk), A.L);
     g.max_wg = max_wg > 0 ? max_wg : 512;
     (void)n_words;
 
@@ -247,20 +255,22 @@ void run_gram(const PrepView& A, const PrepView& B, const GramRun& run, GramScra
     const uint32_t* order = sc.tile_order(tiles_r, tiles_c, run.triangle, s, &n_tiles);
     const uint64_t n_out =
         run.triangle ? (uint64_t)A.n * (A.n - 1) / 2 : (uint64_t)A.n * (uint64_t)run.out_ld;
-    // every launch leaves the region clean: only fresh memory needs zeroing
-    const void* before = sc.flags.get();
-    const size_t before_bytes = sc.flags.size();
+    // every launch sequence leaves the region clean: only an allocation it
+    // has not run in (made here or by a workspace's reserve) needs zeroing
     uint32_t* flags = static_cast<uint32_t*>(
         sc.flags.ensure(flag_region_bytes(n_out, kFlagListCap), "flag bitmap and queue"));
-    if (flags != before || sc.flags.size() != before_bytes)
+    if (sc.flags_clean != sc.flags.generation())
         launch_zero_words(flags, (int64_t)(sc.flags.size() / 16 * 4), s);
+    sc.flags_clean = 0;  // a sequence that throws part-way leaves the region unknown
     run_gram_with(A, B, run, order, n_tiles, flags, kFlagListCap, false, sc.max_wg, s, rec);
+    sc.flags_clean = sc.flags.generation();
 }
 
 void GramScratch::reset()
 {
     if (order_done) (void)hipEventSynchronize(order_done);
     flags.reset();
+    flags_clean = 0;
     for (auto& o : orders) {
         o.buf.reset();
         o.key = ~0ull;

@@ -94,6 +94,8 @@ struct GramThresholds {
     double kappa_dup, u_rep, acc_coef, tau, rep_g_max;
 };
 GramThresholds gram_thresholds(GramMode mode, int L, int n_kblk);
+// The same as the epilogue takes them (scale 1/L, squared tolerances).
+EpilogueLimits epilogue_limits(const GramThresholds& t, int L);
 
 struct GramRun {
     uint32_t* h_count = nullptr;  // pinned: receives the flagged-pair count (async)
@@ -115,6 +117,9 @@ struct GramScratch {
     };
     static constexpr int kOrders = 6;
     DevBuf flags;
+    // flags.generation() of the allocation whose region is known clean
+    // (zeroed here, or left so by a launch sequence that was fully enqueued)
+    uint64_t flags_clean = 0;
     TileOrder orders[kOrders];
     int next_order = 0;
     int max_wg = 0;               // persistent Gram work-groups (2 per CU)

@@ -59,11 +59,7 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
 #pragma unroll
     for (int q = 0; q < kNumConsts; ++q) cj[q] = sconst[(TILE + jl) * 12 + q];
     const int il0 = wr * 32 + 4 * (lane >> 5);  // first row within the tile of this lane
-    const double Ld = (double)a.L;
-    const double invL = 1.0 / Ld;
-    const double tau2L = a.tau * a.tau * Ld;
-    // the solve's own error budget: a quarter of the flagging tolerance
-    const double sec_tol2L = 0.0625 * tau2L;
+    const EpilogueLimits& lim = a.lim;  // uniform: scalar registers
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int il = il0 + 8 * (r / 4) + (r % 4);
@@ -74,25 +70,25 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
         for (int q = 0; q < 9; ++q) E[q] = (double)acc[q][r];
         const SecularSystem y = secular_system(E, ci, cj);
         SecularBracket br = secular_step(y, 0.0);
-        PairSolve o = secular_out(br, sec_tol2L);
+        PairSolve o = secular_out(br, lim.solve_tol2L);
         if (!o.converged & br.pd) {  // rare for a few hundred atoms; taken by waves that need it
             br = secular_step(y, br.lo);
-            o = secular_out(br, sec_tol2L);
+            o = secular_out(br, lim.solve_tol2L);
         }
         const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
         if (ok) {
             const double sgg = ci[10] * cj[10];
-            const double err = a.acc_coef * sgg;
+            const double err = lim.acc_coef * sgg;
             // flag: no verified root, near-duplicate residuals, estimated
             // error above tau (compared squared: no FP64 sqrt), or residuals
             // too large for the split
-            const bool flag = !o.converged || (o.H < a.kappa_dup * a.u_rep * sgg) ||
-                              (err * err > tau2L * o.rmsd2_L) || (2.0 * (ci[9] + cj[9]) > a.rep_g_max);
+            const bool flag = !o.converged || (o.H < lim.dup_coef * sgg) ||
+                              (err * err > lim.tau2L * o.rmsd2_L) || (ci[9] + cj[9] > lim.rep_h_max);
             const int64_t oi = a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
             if (a.out_f64)
-                static_cast<double*>(a.out)[oi] = sqrt(o.rmsd2_L * invL);
-            else
-                static_cast<float*>(a.out)[oi] = sqrtf((float)(o.rmsd2_L * invL));
+                static_cast<double*>(a.out)[oi] = sqrt(o.rmsd2_L * lim.inv_L);
+            else  // v_sqrt_f32 (1 ulp) instead of the 17-instruction correctly rounded sqrtf
+                static_cast<float*>(a.out)[oi] = __builtin_amdgcn_sqrtf((float)(o.rmsd2_L * lim.inv_L));
             flag_output(a, flag, oi);
         }
     }

@@ -84,6 +84,18 @@ void launch_fit_pack(const double* xyz, const double* ref, int n, int n_pad, int
                      GramMode mode, double* rot, double* cent, void* panels, double* consts, int ld,
                      hipStream_t s);
 
+// The epilogue's flagging limits and RMSD scale at one atom count L, formed
+// on the host so that the kernels hold them in scalar registers rather than
+// recompute them per pair.
+struct EpilogueLimits {
+    double inv_L;        // 1 / L
+    double dup_coef;     // flag H < dup_coef sqrt(g_i g_j): near-duplicate residuals
+    double acc_coef;     // estimated error of H - mu per sqrt(g_i g_j)
+    double tau2L;        // tau^2 L, tau the tolerated RMSD error (A)
+    double solve_tol2L;  // the solve's own budget, tau2L / 16
+    double rep_h_max;    // h_i + h_j above which the split's rounding of e may show
+};
+
 struct GramArgs {
     const void* panels_a;   // row operand panels
     const void* panels_b;   // column operand panels
@@ -106,12 +118,7 @@ struct GramArgs {
     int64_t* flag_list;
     uint32_t* flag_count;
     int flag_cap;
-    // flagging thresholds
-    double kappa_dup;       // H < kappa_dup * u_rep * sqrt(g_i g_j)
-    double u_rep;
-    double acc_coef;        // 2^-24 * sqrt(accumulations)
-    double tau;             // tolerated RMSD error estimate (A)
-    double rep_g_max;       // g_i + g_j above which the split's rounding of e may show
+    EpilogueLimits lim;     // flagging limits and the RMSD scale
     // launch shape
     int max_wg;             // resident work-groups (2 per CU)
 };
@@ -168,7 +175,7 @@ struct BatchEns {
     int n, L, n_pad, n_kblk;
     int n_ref;              // conformations sampled for the reference mean
     int n_slices, tile;     // reference-mean slices; tile side 32 or 64
-    double acc_coef, rep_g_max;  // error-model thresholds at this L
+    EpilogueLimits lim;     // error-model limits at this L
 };
 
 // A batch: ensemble table, the prefix sums that map a batch-wide index
@@ -198,7 +205,6 @@ struct BatchArgs {
     uint32_t* ens_bits0;    // the batch-wide bitmap (ensemble 0's bitmap first)
     uint32_t* counters;     // 8 list counters, then the flagged-pair count
     int64_t* flag_list;     // kFlagListCap queued re-evaluations (batch-wide bit indices)
-    double kappa_dup, u_rep, tau;
     int n_wg;               // persistent work-groups of the Gram kernel
 };
 

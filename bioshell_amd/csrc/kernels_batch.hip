@@ -105,11 +105,7 @@ __device__ inline GramArgs ens_args(const BatchArgs& b, const BatchEns& E)
     a.out = E.out;
     a.out_f64 = 0;
     a.flag_bits = E.bits;
-    a.kappa_dup = b.kappa_dup;
-    a.u_rep = b.u_rep;
-    a.acc_coef = E.acc_coef;
-    a.tau = b.tau;
-    a.rep_g_max = E.rep_g_max;
+    a.lim = E.lim;
     return a;
 }
 

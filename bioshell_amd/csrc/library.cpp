@@ -61,8 +61,7 @@ void QueryWorkspace::reserve(int n_q, int n_lib, int L, int k)
     xyz.ensure((size_t)n_q * L * 3 * 8, "query coordinates");
     scores.ensure((size_t)n_q * n_lib * 4, "query scores");
     topk.ensure(topk_bytes(n_q, n_lib, k), "top-k");
-    const uint64_t n_words = (((uint64_t)n_q * n_lib + 31) / 32 + 3) / 4 * 4;
-    gsc.flags.ensure((size_t)(n_words + 4) * 4 + (size_t)kFlagListCap * 8, "flag bitmap and queue");
+    gsc.flags.ensure(flag_region_bytes((uint64_t)n_q * n_lib, kFlagListCap), "flag bitmap and queue");
     gsc.reserve_orders(query_tiles(n_q, n_lib));
 }
 

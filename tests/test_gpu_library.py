@@ -166,6 +166,32 @@ def test_server_large_requests_beside_small_ones(gpu_ctx):
     lib.close()
 
 
+def test_server_slots_start_clean_on_recycled_memory(gpu_ctx):
+    """Slots size their workspaces up front, in memory that may hold anything
+    (here: pages just freed after being filled with set bits).  What that
+    memory held must not queue any pair for FP64 re-evaluation: the
+    re-evaluated value rounds its square root correctly while the epilogue
+    takes v_sqrt_f32, so a spurious re-evaluation shows as a 1-ulp change.
+    Every result equals the synchronous call."""
+    import torch
+
+    lx, qq = make_bucket(141, 200, 64, 32)
+    lib = B.Library(lx, ctx=gpu_ctx)
+    alone = [lib.query_topk(qq[a:a + 16], 10) for a in range(0, 64, 16)]
+    junk = torch.full((64 << 20,), -1, dtype=torch.int32, device="cuda")  # 256 MiB of ones
+    torch.cuda.synchronize()
+    del junk
+    torch.cuda.empty_cache()
+    srv = B.Server([lib], max_inflight=2)
+    tickets = [srv.submit(0, qq[a:a + 16], 10) for a in range(0, 64, 16)]
+    for r, t in enumerate(tickets):
+        idx, val = t.wait()
+        assert np.array_equal(idx, alone[r][0]), r
+        assert np.array_equal(val.view(np.uint32), alone[r][1].view(np.uint32)), r
+    srv.close()
+    lib.close()
+
+
 def test_server_budget_and_cancel(gpu_ctx):
     lx, qq = make_bucket(7, 600, 16, 48)
     lib = B.Library(lx, ctx=gpu_ctx)
