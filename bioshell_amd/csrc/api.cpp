@@ -48,6 +48,41 @@ bool valid_method(bs_method m) { return m >= BS_FP64_PLAIN && m <= BS_BF16X3_GRA
 
 }  // namespace
 
+namespace bs {
+void context_retain(bs_context* ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void context_release(bs_context* ctx)
+{
+    if (ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)guard([&] {
+        DeviceGuard dg(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->prep.reset();
+        ctx->qws.reset();
+        for (auto& r : ctx->ring) r.destroy();
+        ctx->ring.clear();
+        if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+        ctx->h_counts = nullptr;
+        ctx->gsc.reset();
+        for (auto& b : ctx->stream_blk) b.reset();
+        for (int k = 0; k < 2; ++k) {
+            if (ctx->h_blk[k]) (void)hipHostFree(ctx->h_blk[k]);
+            if (ctx->ev_compute[k]) (void)hipEventDestroy(ctx->ev_compute[k]);
+            if (ctx->ev_copy[k]) (void)hipEventDestroy(ctx->ev_copy[k]);
+        }
+        if (ctx->h_blk_counts) (void)hipHostFree(ctx->h_blk_counts);
+        if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+        ctx->ws_xyz.reset();
+        ctx->ws_out.reset();
+        ctx->ws_misc.reset();
+        ctx->ws_stats.reset();
+        ctx->batch.release();
+        (void)hipStreamDestroy(ctx->stream);
+    });
+    delete ctx;
+}
+}  // namespace bs
+
 extern "C" {
 
 const char* bs_last_error(void) { return bs::last_error_cstr(); }
@@ -97,33 +132,7 @@ bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** ou
 
 void bs_context_destroy(bs_context* ctx)
 {
-    if (!ctx) return;
-    (void)guard([&] {
-        DeviceGuard dg(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);
-        ctx->prep.reset();
-        ctx->qws.reset();
-        for (auto& r : ctx->ring) r.destroy();
-        ctx->ring.clear();
-        if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
-        ctx->h_counts = nullptr;
-        ctx->gsc.reset();
-        for (auto& b : ctx->stream_blk) b.reset();
-        for (int k = 0; k < 2; ++k) {
-            if (ctx->h_blk[k]) (void)hipHostFree(ctx->h_blk[k]);
-            if (ctx->ev_compute[k]) (void)hipEventDestroy(ctx->ev_compute[k]);
-            if (ctx->ev_copy[k]) (void)hipEventDestroy(ctx->ev_copy[k]);
-        }
-        if (ctx->h_blk_counts) (void)hipHostFree(ctx->h_blk_counts);
-        if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
-        ctx->ws_xyz.reset();
-        ctx->ws_out.reset();
-        ctx->ws_misc.reset();
-        ctx->ws_stats.reset();
-        ctx->batch.release();
-        (void)hipStreamDestroy(ctx->stream);
-    });
-    delete ctx;
+    if (ctx) context_release(ctx);
 }
 
 bs_status bs_context_synchronize(bs_context* ctx)

@@ -1,6 +1,7 @@
 // engine.hpp -- host orchestration shared by the ABI entry points.
 #pragma once
 
+#include <atomic>
 #include <initializer_list>
 
 #include <vector>
@@ -217,6 +218,9 @@ void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m
 
 // The context: one device, one stream, workspaces, stats, autotune choices.
 struct bs_context {
+    // the creator's reference plus one per library and server made from it:
+    // the context goes when the last of them is destroyed, in any order
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
     bs::Budget budget{0};
@@ -241,3 +245,9 @@ struct bs_context {
     int n_cu = 0;
     std::mutex mu;
 };
+
+namespace bs {
+void context_retain(bs_context* ctx);
+// Drops a reference; the last one frees the context's streams and memory.
+void context_release(bs_context* ctx);
+}  // namespace bs

@@ -103,6 +103,23 @@ void enqueue_query(bs_library* lib, QueryWorkspace& ws, const double* q_host, in
 
 }  // namespace bs
 
+namespace bs {
+void library_retain(bs_library* lib) { lib->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void library_release(bs_library* lib)
+{
+    if (lib->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    bs_context* ctx = lib->ctx;
+    (void)guard([&] {
+        BS_HIP(hipSetDevice(ctx->device));
+        lib->P.reset();
+        lib->staging.reset();
+    });
+    delete lib;
+    context_release(ctx);
+}
+}  // namespace bs
+
 extern "C" {
 
 bs_status bs_library_create(bs_context* ctx, const double* xyz, size_t n_conf, size_t n_atoms,
@@ -129,19 +146,14 @@ bs_status bs_library_create(bs_context* ctx, const double* xyz, size_t n_conf, s
         prepare(lib->P, d, (int)n_conf, (int)n_atoms, kGramBF16x3, nullptr, s);
         BS_HIP(hipStreamSynchronize(s));
         lib->P.partial.reset();
+        context_retain(ctx);
         *out = lib.release();
     });
 }
 
 void bs_library_destroy(bs_library* lib)
 {
-    if (!lib) return;
-    (void)guard([&] {
-        BS_HIP(hipSetDevice(lib->ctx->device));
-        lib->P.reset();
-        lib->staging.reset();
-    });
-    delete lib;
+    if (lib) library_release(lib);
 }
 
 size_t bs_library_size(const bs_library* lib) { return lib ? (size_t)lib->P.n : 0; }

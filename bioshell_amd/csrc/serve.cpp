@@ -393,7 +393,15 @@ bs_status bs_server_create(bs_context* ctx, bs_library* const* libs, size_t n_li
             if (fits) ex->reserve(pre_q, max_n, max_L, pre_k);
             cfg.executors.push_back(std::move(ex));
         }
+        // the server holds a reference to the context and to each library
+        bs_context* c = ctx;
+        cfg.on_close = [c, lv] {
+            for (auto* l : lv) library_release(l);
+            context_release(c);
+        };
         *out = make_server(std::move(cfg));
+        context_retain(ctx);
+        for (auto* l : lv) library_retain(l);
     });
 }
 
@@ -407,7 +415,9 @@ void bs_server_destroy(bs_server* srv)
     srv->cv_work.notify_all();
     if (srv->worker.joinable()) srv->worker.join();  // the worker finishes every queued request first
     srv->slots.clear();  // executors release their streams, workspaces and staging
+    auto on_close = std::move(srv->cfg.on_close);
     delete srv;
+    if (on_close) on_close();
 }
 
 bs_status bs_server_submit(bs_server* srv, size_t lib_index, const double* q_xyz, size_t n_q,

@@ -52,6 +52,8 @@ struct ServerSetup {
     // Whether top-k of k is available for library `lib` (monotone in k).
     std::function<bool(size_t lib, int k)> k_supported;
     std::vector<std::unique_ptr<BatchExecutor>> executors;  // one per slot
+    // Runs when the server is destroyed, after its worker and slots are gone.
+    std::function<void()> on_close;
 };
 
 // Starts a server's worker thread over the given slots.

@@ -105,6 +105,10 @@ int bs_device_count(void);
 /* Creates a context on `device`.  hbm_budget_bytes caps the device memory the
  * context (and libraries / servers made from it) may allocate; 0 = no cap. */
 bs_status bs_context_create(int device, size_t hbm_budget_bytes, bs_context** out);
+/* Gives up the creator's hold on the context.  Libraries and servers made
+ * from it hold it too, so handles may be destroyed in any order: the context
+ * goes when the last of them is destroyed.  No call may use the handle after
+ * its own destroy. */
 void bs_context_destroy(bs_context* ctx);
 bs_status bs_context_synchronize(bs_context* ctx);
 /* The hipStream_t (as void*) on which the context launches its kernels. */
@@ -375,6 +379,8 @@ bs_status bs_ensemble_write_pdb(const bs_ensemble* ens, const double* xyz, size_
 /* A device-resident library of n_conf conformations of n_atoms each. */
 bs_status bs_library_create(bs_context* ctx, const double* xyz, size_t n_conf,
                             size_t n_atoms, bs_library** out);
+/* As bs_context_destroy: a server serving the library keeps it until the
+ * server is destroyed. */
 void bs_library_destroy(bs_library* lib);
 size_t bs_library_size(const bs_library* lib);
 size_t bs_library_atoms(const bs_library* lib);

@@ -192,6 +192,22 @@ def test_server_slots_start_clean_on_recycled_memory(gpu_ctx):
     lib.close()
 
 
+def test_handles_close_in_any_order():
+    """Libraries and servers hold their context (and servers their libraries):
+    closing the context's and the library's handles first leaves the server
+    working, and the memory goes with the last handle."""
+    ctx = B.Context()
+    lx, qq = make_bucket(151, 100, 8, 24)
+    lib = B.Library(lx, ctx=ctx)
+    want = lib.query_topk(qq, 5)
+    srv = B.Server([lib], max_inflight=1)
+    ctx.close()
+    lib.close()
+    idx, val = srv.submit(0, qq, 5).wait()
+    assert np.array_equal(idx, want[0]) and np.array_equal(val, want[1])
+    srv.close()
+
+
 def test_server_budget_and_cancel(gpu_ctx):
     lx, qq = make_bucket(7, 600, 16, 48)
     lib = B.Library(lx, ctx=gpu_ctx)
