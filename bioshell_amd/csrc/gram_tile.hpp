@@ -60,6 +60,7 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
     for (int q = 0; q < kNumConsts; ++q) cj[q] = sconst[(TILE + jl) * 12 + q];
     const int il0 = wr * 32 + 4 * (lane >> 5);  // first row within the tile of this lane
     const EpilogueLimits& lim = a.lim;  // uniform: scalar registers
+    uint64_t flagged[16];               // lanes whose pair r goes to FP64 re-evaluation
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int il = il0 + 8 * (r / 4) + (r % 4);
@@ -89,7 +90,22 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
                 static_cast<double*>(a.out)[oi] = sqrt(o.rmsd2_L * lim.inv_L);
             else  // v_sqrt_f32 (1 ulp) instead of the 17-instruction correctly rounded sqrtf
                 static_cast<float*>(a.out)[oi] = __builtin_amdgcn_sqrtf((float)(o.rmsd2_L * lim.inv_L));
-            flag_output(a, flag, oi);
+            flagged[r] = __builtin_amdgcn_ballot_w64(flag);
+        } else {
+            flagged[r] = 0;
+        }
+    }
+    // flagged pairs are rare: one branch per tile rather than a queueing
+    // branch per pair (those cost about a sixth of the epilogue)
+    uint64_t any = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) any |= flagged[r];
+    if (any) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            if (!((flagged[r] >> lane) & 1u)) continue;
+            const int i = tr * TILE + il0 + 8 * (r / 4) + (r % 4);
+            flag_output(a, true, a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j);
         }
     }
 }
@@ -99,6 +115,8 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
 // form also waits for every LDS access of the wave (lgkmcnt 0).
 #define BS_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(0x0F70 | ((N) & 15) | (((N) >> 4) << 14))
 #define BS_WAIT_VMCNT_LDS0(N) __builtin_amdgcn_s_waitcnt(0x0070 | ((N) & 15) | (((N) >> 4) << 14))
+// lgkmcnt(0) alone
+#define BS_WAIT_LDS0() __builtin_amdgcn_s_waitcnt(0xC07F)
 
 // Direct global -> LDS 16-byte loads: lane l of the wave writes dst + 16 l.
 #define BS_GLOAD_LDS(SRC, DST)                                                                  \
@@ -174,21 +192,78 @@ __device__ __forceinline__ void mfma_regs(const OperandRegs& r, f32x16 (&acc)[9]
     }
 }
 
+// BF16x3 operands of one atom block for one wave, one plane (0: hi, 1: lo):
+// words 2c + plane of its row block (a) and of its column block (b).
+struct PlaneRegs {
+    uint4 a[3], b[3];
+};
+
+__device__ __forceinline__ void load_plane(const uint4* __restrict__ A, const uint4* __restrict__ B, int lane,
+                                           int plane, PlaneRegs& r)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r.a[c] = A[(2 * c + plane) * 64 + lane];
+        r.b[c] = B[(2 * c + plane) * 64 + lane];
+    }
+}
+
+// The hi * hi products of one atom block (the first 9 of mfma_regs' 27).
+__device__ __forceinline__ void mfma_hh(const PlaneRegs& h, f32x16 (&acc)[9])
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            acc[3 * c + d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(h.a[c]), as_bf16x8(h.b[d]),
+                                                                     acc[3 * c + d], 0, 0, 0);
+}
+
+// Cross product m (0..17) of one atom block: hi * lo for m < 9, then lo * hi,
+// in mfma_regs' order, so the accumulation order is the same.
+__device__ __forceinline__ void mfma_cross(const PlaneRegs& h, const PlaneRegs& l, f32x16 (&acc)[9], int m)
+{
+    const int q = m % 9, c = q / 3, d = q % 3;
+    if (m < 9)
+        acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(h.a[c]), as_bf16x8(l.b[d]), acc[q], 0, 0, 0);
+    else
+        acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(l.a[c]), as_bf16x8(h.b[d]), acc[q], 0, 0, 0);
+}
+
 // One 64 x 64 tile of pairs by a 4-wave work-group.  K loop over atom
 // blocks with a three-stage LDS ring fed by direct global->LDS loads (no
 // registers held for staging): a stage holds the four operand blocks (A0,
-// A1, B0, B1: 6 KiB each) of one atom block; wave w streams block w.  The
-// operands of block k are read from LDS into registers during step k-1, so
-// the MFMAs of a step never wait on LDS.  Step k waits for its own loads of
-// block k+1 (vmcnt: the 6 loads of block k+2 may still be in flight) and for
-// its own LDS reads of block k (lgkmcnt 0), meets the others at a barrier
-// (after which every wave has block k in registers and block k+1 in LDS),
-// issues block k+3 into the stage block k came from, reads block k+1 into the
-// other register set, and runs the 27 (or 72) MFMAs of block k.  Steps are
-// unrolled by six so that every LDS stage and register set is named
-// statically.  Loads past the last block re-read the last block (in bounds,
-// unused); the loop drains them before the epilogue reuses the LDS.
-template <int MODE>
+// A1, B0, B1: 6 KiB each) of one atom block; wave w streams block w.  Loads
+// past the last block re-read the last block (in bounds, unused); the loop
+// drains them before the epilogue reuses the LDS.  Steps are unrolled by six
+// so that every LDS stage and register set is named statically.
+//
+// BF16x3.  Only the hi plane of the next block is read ahead (24 VGPRs; a
+// whole second operand set does not fit beside 144 accumulators, and the
+// compiler then sinks the reads to their use, exposing LDS latency).  Step k
+// holds hi_k in registers and:
+//   reads lo_k from its stage, and runs the 9 hi*hi products (which cover
+//   those reads);
+//   waits for its own loads of block k+1 (vmcnt: block k+2's 6 may still be
+//   in flight) and its LDS reads, and meets the others at a barrier: block k
+//   is then in every wave's registers and block k+1 in LDS;
+//   reads hi_{k+1}, then issues block k+3 into block k's stage one load per
+//   3 of the 18 cross products (back to back, the loads' setup stalled the
+//   MFMA stream: 4 % of the launch);
+//   waits for hi_{k+1} (lgkmcnt 0, long landed).  The explicit wait matters:
+//   while LDS-DMA loads are in flight the compiler turns any LDS-result wait
+//   into lgkmcnt(0), which before the hi*hi products would also wait for
+//   lo_k just issued.
+// The LDS reads come before the DMA loads in each group: read after DMA, the
+// compiler cannot tell the two apart and waits vmcnt(0).
+//
+// FP32, and BF16x3 with SPLIT false (the persistent batch kernel, whose
+// own loop state leaves no room for the split schedule: it would spill).
+// The operands of block k are read into registers during step k-1; step k
+// waits for its own loads of block k+1 and its LDS reads, meets the others
+// at a barrier, issues block k+3 into block k's stage, reads block k+1 into
+// the other register set, and runs the 27 (or 72) MFMAs of block k.
+template <int MODE, bool SPLIT>
 __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
                                             uint4 (&lds)[3][4][kPanelBlockU4], int thread)
 {
@@ -218,6 +293,34 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
         mfma_regs<MODE>(RCUR, acc);                                               \
     }
 
+#define BS_STEP3(KB, SCUR, SNEXT, HCUR, HNEXT)                                    \
+    {                                                                             \
+        PlaneRegs lo_;                                                            \
+        load_plane(lds[SCUR][wr], lds[SCUR][2 + wc], lane, 1, lo_);               \
+        __builtin_amdgcn_sched_barrier(0);                                        \
+        mfma_hh(HCUR, acc);                                                       \
+        __builtin_amdgcn_sched_barrier(0);                                        \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                    \
+        BS_WAIT_VMCNT_LDS0(6);                                                    \
+        __builtin_amdgcn_s_barrier();                                             \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                    \
+        load_plane(lds[SNEXT][wr], lds[SNEXT][2 + wc], lane, 0, HNEXT);           \
+        __builtin_amdgcn_sched_barrier(0);                                        \
+        {                                                                         \
+            const uint4* g_ = src + (int64_t)min((KB) + 3, last) * kPanelBlockU4;  \
+            _Pragma("unroll") for (int u_ = 0; u_ < 6; ++u_) {                    \
+                BS_GLOAD_LDS(g_ + u_ * 64, &lds[SCUR][wv][u_ * 64]);              \
+                __builtin_amdgcn_sched_barrier(0);                                \
+                mfma_cross(HCUR, lo_, acc, 3 * u_);                               \
+                mfma_cross(HCUR, lo_, acc, 3 * u_ + 1);                           \
+                mfma_cross(HCUR, lo_, acc, 3 * u_ + 2);                           \
+                __builtin_amdgcn_sched_barrier(0);                                \
+            }                                                                     \
+        }                                                                         \
+        BS_WAIT_LDS0();                                                           \
+        __builtin_amdgcn_sched_barrier(0);                                        \
+    }
+
     f32x16 acc[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q)
@@ -228,31 +331,52 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
     // normal one: with two work-groups per CU in different phases, the
     // sequencer then favours the MFMA stream
     __builtin_amdgcn_s_setprio(2);
-    OperandRegs ra, rb;
     BS_ISSUE(0, 0)
     BS_ISSUE(1, 1)
     BS_ISSUE(2, 2)
     BS_WAIT_VMCNT(12);  // own loads of block 0 landed
     __syncthreads();
-    load_operands<MODE>(lds[0][wr], lds[0][2 + wc], lane, ra);
-    for (int kb = 0;; kb += 6) {
-        BS_STEP(kb, 0, 1, ra, rb)
-        if (kb + 1 >= n_kblk) break;
-        BS_STEP(kb + 1, 1, 2, rb, ra)
-        if (kb + 2 >= n_kblk) break;
-        BS_STEP(kb + 2, 2, 0, ra, rb)
-        if (kb + 3 >= n_kblk) break;
-        BS_STEP(kb + 3, 0, 1, rb, ra)
-        if (kb + 4 >= n_kblk) break;
-        BS_STEP(kb + 4, 1, 2, ra, rb)
-        if (kb + 5 >= n_kblk) break;
-        BS_STEP(kb + 5, 2, 0, rb, ra)
-        if (kb + 6 >= n_kblk) break;
+    if constexpr (MODE == kGramBF16x3 && SPLIT) {
+        PlaneRegs ha, hb;
+        load_plane(lds[0][wr], lds[0][2 + wc], lane, 0, ha);
+        BS_WAIT_LDS0();
+        for (int kb = 0;; kb += 6) {
+            BS_STEP3(kb, 0, 1, ha, hb)
+            if (kb + 1 >= n_kblk) break;
+            BS_STEP3(kb + 1, 1, 2, hb, ha)
+            if (kb + 2 >= n_kblk) break;
+            BS_STEP3(kb + 2, 2, 0, ha, hb)
+            if (kb + 3 >= n_kblk) break;
+            BS_STEP3(kb + 3, 0, 1, hb, ha)
+            if (kb + 4 >= n_kblk) break;
+            BS_STEP3(kb + 4, 1, 2, ha, hb)
+            if (kb + 5 >= n_kblk) break;
+            BS_STEP3(kb + 5, 2, 0, hb, ha)
+            if (kb + 6 >= n_kblk) break;
+        }
+    } else {
+        OperandRegs ra, rb;
+        load_operands<MODE>(lds[0][wr], lds[0][2 + wc], lane, ra);
+        for (int kb = 0;; kb += 6) {
+            BS_STEP(kb, 0, 1, ra, rb)
+            if (kb + 1 >= n_kblk) break;
+            BS_STEP(kb + 1, 1, 2, rb, ra)
+            if (kb + 2 >= n_kblk) break;
+            BS_STEP(kb + 2, 2, 0, ra, rb)
+            if (kb + 3 >= n_kblk) break;
+            BS_STEP(kb + 3, 0, 1, rb, ra)
+            if (kb + 4 >= n_kblk) break;
+            BS_STEP(kb + 4, 1, 2, ra, rb)
+            if (kb + 5 >= n_kblk) break;
+            BS_STEP(kb + 5, 2, 0, rb, ra)
+            if (kb + 6 >= n_kblk) break;
+        }
     }
     BS_WAIT_VMCNT(0);  // no load may still write the LDS the epilogue reuses
     __syncthreads();
 #undef BS_ISSUE
 #undef BS_STEP
+#undef BS_STEP3
 
     __builtin_amdgcn_s_setprio(0);
     epilogue<64>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
