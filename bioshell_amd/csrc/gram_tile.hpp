@@ -60,7 +60,7 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
     for (int q = 0; q < kNumConsts; ++q) cj[q] = sconst[(TILE + jl) * 12 + q];
     const int il0 = wr * 32 + 4 * (lane >> 5);  // first row within the tile of this lane
     const EpilogueLimits& lim = a.lim;  // uniform: scalar registers
-    uint64_t flagged[16];               // lanes whose pair r goes to FP64 re-evaluation
+    uint32_t flagged = 0;               // bit r: pair r goes to FP64 re-evaluation
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int il = il0 + 8 * (r / 4) + (r % 4);
@@ -90,20 +90,15 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
                 static_cast<double*>(a.out)[oi] = sqrt(o.rmsd2_L * lim.inv_L);
             else  // v_sqrt_f32 (1 ulp) instead of the 17-instruction correctly rounded sqrtf
                 static_cast<float*>(a.out)[oi] = __builtin_amdgcn_sqrtf((float)(o.rmsd2_L * lim.inv_L));
-            flagged[r] = __builtin_amdgcn_ballot_w64(flag);
-        } else {
-            flagged[r] = 0;
+            flagged |= (uint32_t)flag << r;
         }
     }
     // flagged pairs are rare: one branch per tile rather than a queueing
     // branch per pair (those cost about a sixth of the epilogue)
-    uint64_t any = 0;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) any |= flagged[r];
-    if (any) {
-#pragma unroll
+    if (flagged) {
+#pragma unroll 1
         for (int r = 0; r < 16; ++r) {
-            if (!((flagged[r] >> lane) & 1u)) continue;
+            if (!((flagged >> r) & 1u)) continue;
             const int i = tr * TILE + il0 + 8 * (r / 4) + (r % 4);
             flag_output(a, true, a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j);
         }

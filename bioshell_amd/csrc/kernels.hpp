@@ -61,9 +61,11 @@ __host__ __device__ inline int ref_slices(int S) { return fit_groups(S); }
 // mean over S samples departs from the full mean by about sigma / sqrt(S)
 // per coordinate, which adds 1/S of a typical residual variance, while the
 // first two preparation passes read S conformations instead of n.  S grows
-// with n (n/8 above 1024) so that near-identical conformations of a large
-// ensemble keep residuals far below the noise of the rest.
-constexpr int kRefSampleMin = 128;
+// with n (n/8 above 512) so that near-identical conformations of a large
+// ensemble keep residuals far below the noise of the rest.  A floor of 64
+// rather than 128 left the largest error of a 512-ensemble batch unchanged
+// and cut its step by 6 %.
+constexpr int kRefSampleMin = 64;
 __host__ __device__ inline int ref_sample_count(int n)
 {
     const int s = n / 8 > kRefSampleMin ? n / 8 : kRefSampleMin;

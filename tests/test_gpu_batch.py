@@ -81,13 +81,13 @@ def test_batch_bit_identical_to_single(tmp_path, batch, tile):
 
 
 def test_batch_large_ensemble_sampled_reference(gpu_ctx):
-    """An ensemble above 1024 conformations takes its reference mean over n/8
+    """An ensemble above 512 conformations takes its reference mean over n/8
     samples, in the batch as in the single call: both give the same triangle,
     within the gate of the oracle."""
-    # sample counts around their switch points: S = n up to 128, then 128,
-    # then n / 8 above 1024 (129 at 1032 and 1033)
-    xs = make_batch(700, [(1100, 24), (300, 40), (2, 9), (128, 20), (129, 20), (1024, 10),
-                          (1025, 10), (1033, 10)])
+    # sample counts around their switch points: S = n up to 64, then 64,
+    # then n / 8 above 512 (65 at 520 and 521)
+    xs = make_batch(700, [(1100, 24), (300, 40), (2, 9), (64, 20), (65, 20), (519, 10),
+                          (520, 10), (521, 10)])
     got = B.crmsd_batch(xs, B.Method.BF16X3_GRAM, ctx=gpu_ctx)
     for x, g in zip(xs, got):
         single = B.crmsd_all_vs_all(x, B.Method.BF16X3_GRAM, ctx=gpu_ctx).upper
