@@ -270,12 +270,15 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
     const uint4* src = (wv < 2 ? pa + (int64_t)(2 * tr + wv) * stride_cb
                                : pb + (int64_t)(2 * tc + wv - 2) * stride_cb) + lane;
     const int n_kblk = a.n_kblk, last = n_kblk - 1;
+    // the wave index as a uniform value: the loads' LDS targets (M0) are then
+    // formed in SGPRs, not by a v_readfirstlane per load (1-2 % of the launch)
+    const int wvu = __builtin_amdgcn_readfirstlane(wv);
 
 #define BS_ISSUE(KB, ST)                                                          \
     {                                                                             \
         const uint4* g_ = src + (int64_t)min((KB), last) * kPanelBlockU4;         \
         _Pragma("unroll") for (int u_ = 0; u_ < 6; ++u_)                          \
-            BS_GLOAD_LDS(g_ + u_ * 64, &lds[ST][wv][u_ * 64]);                    \
+            BS_GLOAD_LDS(g_ + u_ * 64, &lds[ST][wvu][u_ * 64]);                   \
     }
 #define BS_STEP(KB, SCUR, SNEXT, RCUR, RNEXT)                                     \
     {                                                                             \
@@ -304,7 +307,7 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
         {                                                                         \
             const uint4* g_ = src + (int64_t)min((KB) + 3, last) * kPanelBlockU4;  \
             _Pragma("unroll") for (int u_ = 0; u_ < 6; ++u_) {                    \
-                BS_GLOAD_LDS(g_ + u_ * 64, &lds[SCUR][wv][u_ * 64]);              \
+                BS_GLOAD_LDS(g_ + u_ * 64, &lds[SCUR][wvu][u_ * 64]);             \
                 __builtin_amdgcn_sched_barrier(0);                                \
                 mfma_cross(HCUR, lo_, acc, 3 * u_);                               \
                 mfma_cross(HCUR, lo_, acc, 3 * u_ + 1);                           \
