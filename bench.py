@@ -254,6 +254,25 @@ def synthetic_cif(n_models: int) -> bytes:
     return "".join(out).encode()
 
 
+def stream_ms(torch, ctx, fn, reps: int) -> tuple[float, float]:
+    """(device ms, host ms) per call of fn over reps calls, after one untimed
+    call.  Device time is HIP events recorded on the context's own stream, the
+    stream the library launches on (torch's current stream would see none of it);
+    host time also holds the launch overhead of each call."""
+    s = torch.cuda.ExternalStream(ctx.stream)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    b.synchronize()
+    host = (time.perf_counter() - t0) * 1e3 / reps
+    return a.elapsed_time(b) / reps, host
+
+
 def next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x) -> dict:
     """The path's neighbours, each on its own workload: clustering and row
     reductions over the headline triangle already in HBM, ensemble superposition
@@ -275,16 +294,13 @@ def next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x) -> dict:
     from bioshell_amd import _native
 
     vp = ctypes.c_void_p
-    B.check(_native.lib().bs_rmsd_row_stats_device(ctx.handle, vp(d_out.data_ptr()), n, vp(d_lo.data_ptr()),
-                                                    vp(d_hi.data_ptr())))
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    reps = 10
-    for _ in range(reps):
+    reps = 20
+
+    def row_stats():
         B.check(_native.lib().bs_rmsd_row_stats_device(ctx.handle, vp(d_out.data_ptr()), n, vp(d_lo.data_ptr()),
                                                         vp(d_hi.data_ptr())))
-    ctx.synchronize()
-    rs_ms = (time.perf_counter() - t0) * 1e3 / reps
+
+    rs_ms, rs_host_ms = stream_ms(torch, ctx, row_stats, reps)
     tri_bytes = n * (n - 1) // 2 * 4
     cn = min(n, args.cluster_n)
     d_tri = d_out
@@ -348,6 +364,7 @@ def next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x) -> dict:
                          "sample": "oracle or_hclust (C restatement of hierarchical.rs:22-80) on the first "
                                    "1000 elements of the same matrix"},
         "row_stats_ms": round(rs_ms, 4),
+        "row_stats_host_ms": round(rs_host_ms, 4),
         "row_stats_roofline": {"bound": "hbm", "achieved": round(tri_bytes / (rs_ms * 1e-3) / 1e9, 1),
                                "peak": 8000.0, "unit": "GB/s",
                                "frac": round(tri_bytes / (rs_ms * 1e-3) / 8e12, 4), "traffic": None},
@@ -363,13 +380,7 @@ def next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x) -> dict:
         B.superpose_ensemble_device(d_xyz.data_ptr(), n, L, None, 0, d_rot.data_ptr(), d_tr.data_ptr(),
                                     d_rm.data_ptr(), d_sp.data_ptr(), ctx=ctx)
 
-    sup()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        sup()
-    ctx.synchronize()
-    sp_ms = (time.perf_counter() - t0) * 1e3 / reps
+    sp_ms, sp_host_ms = stream_ms(torch, ctx, sup, reps)
     algo = n * L * 24 * 2  # coordinates read once, superposed coordinates written once
     rm = d_rm.cpu().numpy()
     k = 300
@@ -380,6 +391,7 @@ def next_rows_leg(args, B, torch, log, ctx, d_xyz, d_out, x) -> dict:
         "workload": f"{n} x {L} ensemble fitted onto conformation 0: R, t, RMSD and the superposed "
                     "coordinates, FP64, in HBM (bs_superpose_ensemble_device)",
         "ms": round(sp_ms, 4),
+        "host_ms": round(sp_host_ms, 4),
         "value": round(n / (sp_ms * 1e-3), 1),
         "unit": "conformations/s",
         "roofline": {"bound": "hbm", "achieved": round(algo / (sp_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
