@@ -232,10 +232,9 @@ void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, con
     if (rec) rec->mark({kBMain1, kBFallback0}, s);
     BS_HIP(hipGetLastError());
     launch_flag_resolve(A.xyz, A.cent, B.xyz, B.cent, A.L, ctl, list, list_cap, bits, (int64_t)n_out,
-                        g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, s);
+                        g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, rec ? rec->h_count : nullptr,
+                        run.h_count, s);
     BS_HIP(hipGetLastError());
-    if (rec && rec->h_count) BS_HIP(hipMemcpyAsync(rec->h_count, ctl + 1, 4, hipMemcpyDeviceToHost, s));
-    if (run.h_count) BS_HIP(hipMemcpyAsync(run.h_count, ctl + 1, 4, hipMemcpyDeviceToHost, s));
     if (rec) {
         if (run.ends_total)
             rec->mark({kBFallback1, kBTotal1}, s);

@@ -156,7 +156,7 @@ struct RunRecord {
     hipEvent_t ev[8] = {};        // in the order recorded
     int8_t slot[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // boundary -> index into ev
     int n_ev = 0;
-    uint32_t* h_count = nullptr;  // flagged-pair count (pinned), copied after the call
+    uint32_t* h_count = nullptr;  // flagged-pair count (pinned), stored by the resolve kernel
     bs_run_stats base{};          // pairs, launches, tile sizes
     bool gram = false;
     bool used = false;
@@ -169,7 +169,7 @@ struct RunRecord {
 
 // Gram contraction + QCP epilogue + FP64 re-evaluation of flagged pairs over
 // rows of A x cols of B (or the upper triangle of A when triangle).  Asynchronous:
-// the flagged-pair count is copied to rec->h_count when rec is given.
+// the flagged-pair count is stored to rec->h_count when rec is given.
 // Words of the flag bitmap of n_out outputs (a whole number of 16-byte units).
 inline uint64_t flag_words(uint64_t n_out) { return ((n_out + 31) / 32 + 3) / 4 * 4; }
 // Bytes of the flag region: four control words (count, report, done, spare),

@@ -137,11 +137,13 @@ void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 // `list` (min(count, cap) entries), then, only when count > cap, a scan of the
 // bitmap `bits` (n_out bits) that clears every word it reads.  ctl holds
 // count, report, done: the last work-group to finish copies count to report
-// and zeroes count and done, so the region is ready for the next launch.
+// and zeroes count and done, so the region is ready for the next launch.  It
+// also stores the count into h_report_a / h_report_b when given (pinned host
+// words, written with a system-scope store: no separate device-to-host copy).
 void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double* xyz_b,
                          const double* cent_b, int L, uint32_t* ctl, const int64_t* list, int cap,
                          uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld, void* out,
-                         int out_f64, hipStream_t s);
+                         int out_f64, uint32_t* h_report_a, uint32_t* h_report_b, hipStream_t s);
 // Queues the set bits of bits[0 .. n_words) as bit indices (word * 32 + bit)
 // into list (cap entries), clearing the queued ones; *count += set bits.
 void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,

@@ -144,7 +144,8 @@ __global__ __launch_bounds__(256) void flag_resolve_kernel(const double* __restr
                                                            const int64_t* __restrict__ list, int cap,
                                                            uint32_t* __restrict__ bits, int64_t n_out,
                                                            int triangle, int n, int64_t out_ld,
-                                                           void* __restrict__ out, int out_f64)
+                                                           void* __restrict__ out, int out_f64,
+                                                           uint32_t* h_report_a, uint32_t* h_report_b)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t cnt = ctl[0];  // read before this group counts itself done
@@ -181,6 +182,8 @@ __global__ __launch_bounds__(256) void flag_resolve_kernel(const double* __restr
         ctl[1] = cnt;
         ctl[0] = 0u;
         ctl[2] = 0u;
+        if (h_report_a) __hip_atomic_store(h_report_a, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (h_report_b) __hip_atomic_store(h_report_b, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -225,13 +228,14 @@ void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s)
 void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double* xyz_b,
                          const double* cent_b, int L, uint32_t* ctl, const int64_t* list, int cap,
                          uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld, void* out,
-                         int out_f64, hipStream_t s)
+                         int out_f64, uint32_t* h_report_a, uint32_t* h_report_b, hipStream_t s)
 {
     // enough waves for a full list or a bitmap scan; idle groups leave at once
     const int64_t n_words = (n_out + 31) / 32;
     const int64_t blocks = std::min<int64_t>(1024, std::max<int64_t>(1, (n_words + 255) / 256));
     flag_resolve_kernel<<<(int)std::max<int64_t>(blocks, 256), 256, 0, s>>>(
-        xyz_a, cent_a, xyz_b, cent_b, L, ctl, list, cap, bits, n_out, triangle, n, out_ld, out, out_f64);
+        xyz_a, cent_a, xyz_b, cent_b, L, ctl, list, cap, bits, n_out, triangle, n, out_ld, out, out_f64,
+        h_report_a, h_report_b);
 }
 
 void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,
