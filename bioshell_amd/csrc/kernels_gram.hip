@@ -230,10 +230,11 @@ void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double
                          uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld, void* out,
                          int out_f64, uint32_t* h_report_a, uint32_t* h_report_b, hipStream_t s)
 {
-    // enough waves for a full list or a bitmap scan; idle groups leave at once
-    const int64_t n_words = (n_out + 31) / 32;
-    const int64_t blocks = std::min<int64_t>(1024, std::max<int64_t>(1, (n_words + 255) / 256));
-    flag_resolve_kernel<<<(int)std::max<int64_t>(blocks, 256), 256, 0, s>>>(
+    // 1024 waves: a typical list (tens to thousands of pairs) is one pass, a
+    // full list or a bitmap scan takes several.  A grid sized for the bitmap
+    // (up to 1024 groups) cost 9 us more per call in dispatch and in the
+    // done-counter atomics of the idle groups.
+    flag_resolve_kernel<<<256, 256, 0, s>>>(
         xyz_a, cent_a, xyz_b, cent_b, L, ctl, list, cap, bits, n_out, triangle, n, out_ld, out, out_f64,
         h_report_a, h_report_b);
 }

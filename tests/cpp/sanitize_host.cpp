@@ -16,6 +16,7 @@
 //            budget admits batches one at a time.
 //   gpu      (with --gpu) the default context from several threads, and a
 //            device server with concurrent submit / release / wait.
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -119,7 +120,11 @@ void section_tsv()
     std::mt19937 rng(3);
     std::uniform_real_distribution<float> U(0.0f, 30.0f);
     for (auto& v : up) v = U(rng);
-    const char* path = "sanitize_host.tsv";
+    // One file per process: the ASan and TSan builds may run at the same time.
+    const char* tmp = std::getenv("TMPDIR");
+    const std::string file = std::string(tmp && *tmp ? tmp : "/tmp") + "/sanitize_host_" +
+                             std::to_string(getpid()) + ".tsv";
+    const char* path = file.c_str();
     CHECK(bs_write_distance_tsv(up.data(), n, nullptr, -1, 0, 16, path) == BS_OK);
     std::ifstream in(path);
     std::string a, b;
