@@ -18,8 +18,7 @@ constexpr int epilogue_lds_bytes()
 // 32 (w/2) .. +31 and columns 32 (w%2) .. +31), TILE 32 is one wave on its own.
 // The rows' and columns' constants go to LDS at `lds` (which the caller no
 // longer needs); each lane then solves its 16 pairs straight from its
-// accumulator registers, all 16 unrolled so that their independent FP64
-// chains interleave.  Accumulator slot r of a lane holds row
+// accumulator registers, P at a time.  Accumulator slot r of a lane holds row
 // 8 (r/4) + 4 (lane/32) + r%4 of the wave's block and column lane % 32.
 //
 // The solve is Newton on the secular form of the QCP polynomial from mu = 0
@@ -27,10 +26,11 @@ constexpr int epilogue_lds_bytes()
 // a pair whose S = -B is not positive definite or whose bracket stays wide
 // is flagged, like every pair the error model cannot vouch for, and is
 // re-evaluated in FP64 from the coordinates.
-template <int TILE>
+template <int TILE, int P>
 __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x16 (&acc)[9],
                                          char* lds, int thread)
 {
+    static_assert(P == 1 || P == 2, "pairs per block: 1 or 2 (rows of one group of four)");
     constexpr int kWaves = (TILE / 32) * (TILE / 32);
     constexpr int kThreads = 64 * kWaves;
     const int lane = thread & 63;
@@ -61,36 +61,67 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
     const int il0 = wr * 32 + 4 * (lane >> 5);  // first row within the tile of this lane
     const EpilogueLimits& lim = a.lim;  // uniform: scalar registers
     uint32_t flagged = 0;               // bit r: pair r goes to FP64 re-evaluation
+    // P = 2 solves pairs r and r + 1 (rows il and il + 1) together: with both
+    // rows' constants read from LDS at once and no branch between their two
+    // solves, the scheduler interleaves the two independent FP64 chains, where
+    // one pair per block leaves each chain's latency exposed.  It needs about
+    // a dozen more registers than P = 1; four pairs per block spill while the
+    // accumulators of the later pairs are still live.
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int il = il0 + 8 * (r / 4) + (r % 4);
+    for (int r = 0; r < 16; r += P) {
+        const int il = il0 + 8 * (r / 4) + (r % 4);  // P = 2: r % 4 is 0 or 2, row il + 1 holds pair r + 1
         const int i = tr * TILE + il;
-        const double* ci = sconst + il * 12;  // read from LDS where used (broadcast)
-        double E[9];
+        double c[P][kNumConsts];
+        SecularSystem y[P];
+        SecularBracket br[P];
+        PairSolve o[P];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) E[q] = (double)acc[q][r];
-        const SecularSystem y = secular_system(E, ci, cj);
-        SecularBracket br = secular_step(y, 0.0);
-        PairSolve o = secular_out(br, lim.solve_tol2L);
-        if (!o.converged & br.pd) {  // rare for a few hundred atoms; taken by waves that need it
-            br = secular_step(y, br.lo);
-            o = secular_out(br, lim.solve_tol2L);
+        for (int p = 0; p < P; ++p) {
+#pragma unroll
+            for (int q = 0; q < kNumConsts; ++q) c[p][q] = sconst[(il + p) * 12 + q];
         }
-        const bool ok = jok && i < a.n_rows && (!a.triangle || i < j);
-        if (ok) {
-            const double sgg = ci[10] * cj[10];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            double E[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) E[q] = (double)acc[q][r + p];
+            y[p] = secular_system(E, c[p], cj);
+        }
+        bool again = false;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            br[p] = secular_step(y[p], 0.0);
+            o[p] = secular_out(br[p], lim.solve_tol2L);
+            again |= !o[p].converged & br[p].pd;
+        }
+        if (again) {  // rare for a few hundred atoms; taken by waves that need it
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                if (!o[p].converged & br[p].pd) {
+                    br[p] = secular_step(y[p], br[p].lo);
+                    o[p] = secular_out(br[p], lim.solve_tol2L);
+                }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const bool ok = jok && i + p < a.n_rows && (!a.triangle || i + p < j);
+            const double sgg = c[p][10] * cj[10];
             const double err = lim.acc_coef * sgg;
             // flag: no verified root, near-duplicate residuals, estimated
             // error above tau (compared squared: no FP64 sqrt), or residuals
             // too large for the split
-            const bool flag = !o.converged || (o.H < lim.dup_coef * sgg) ||
-                              (err * err > lim.tau2L * o.rmsd2_L) || (ci[9] + cj[9] > lim.rep_h_max);
-            const int64_t oi = a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j;
-            if (a.out_f64)
-                static_cast<double*>(a.out)[oi] = sqrt(o.rmsd2_L * lim.inv_L);
-            else  // v_sqrt_f32 (1 ulp) instead of the 17-instruction correctly rounded sqrtf
-                static_cast<float*>(a.out)[oi] = __builtin_amdgcn_sqrtf((float)(o.rmsd2_L * lim.inv_L));
-            flagged |= (uint32_t)flag << r;
+            const bool flag = !o[p].converged || (o[p].H < lim.dup_coef * sgg) ||
+                              (err * err > lim.tau2L * o[p].rmsd2_L) || (c[p][9] + cj[9] > lim.rep_h_max);
+            const int64_t oi =
+                a.triangle ? tri_index(a.n_rows, i + p, j) : (int64_t)(i + p) * a.out_ld + j;
+            if (ok) {
+                if (a.out_f64)
+                    static_cast<double*>(a.out)[oi] = sqrt(o[p].rmsd2_L * lim.inv_L);
+                else  // v_sqrt_f32 (1 ulp) instead of the 17-instruction correctly rounded sqrtf
+                    static_cast<float*>(a.out)[oi] =
+                        __builtin_amdgcn_sqrtf((float)(o[p].rmsd2_L * lim.inv_L));
+            }
+            flagged |= (uint32_t)(ok & flag) << (r + p);
         }
     }
     // flagged pairs are rare: one branch per tile rather than a queueing
@@ -258,7 +289,10 @@ __device__ __forceinline__ void mfma_cross(const PlaneRegs& h, const PlaneRegs& 
 // waits for its own loads of block k+1 and its LDS reads, meets the others
 // at a barrier, issues block k+3 into block k's stage, reads block k+1 into
 // the other register set, and runs the 27 (or 72) MFMAs of block k.
-template <int MODE, bool SPLIT>
+//
+// PAIRS: pairs per block of the epilogue, 2 where the registers allow (the
+// Gram kernel), 1 in the batch kernel.
+template <int MODE, bool SPLIT, int PAIRS>
 __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
                                             uint4 (&lds)[3][4][kPanelBlockU4], int thread)
 {
@@ -377,7 +411,7 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
 #undef BS_STEP3
 
     __builtin_amdgcn_s_setprio(0);
-    epilogue<64>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
+    epilogue<64, PAIRS>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
 }
 
 // One 32 x 32 tile of pairs by a single wave: operand fragments are read
@@ -403,7 +437,7 @@ __device__ __forceinline__ void gram_tile32(const GramArgs& a, int tr, int tc, c
     for (int kb = 0; kb < a.n_kblk; ++kb)
         mfma_block<MODE>(A + (int64_t)kb * kPanelBlockU4, B + (int64_t)kb * kPanelBlockU4, lane, acc);
     __builtin_amdgcn_s_setprio(0);
-    epilogue<32>(a, tr, tc, acc, lds, thread);
+    epilogue<32, 1>(a, tr, tc, acc, lds, thread);
 }
 
 }  // namespace bs

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256, 2) void batch_gram_kernel(BatchArgs b)
             const uint64_t code = uniform64(b.tiles64[item]);
             const BatchEns& E = b.ens[(int)(code >> 32)];
             const GramArgs a = ens_args(b, E);
-            gram_tile64<MODE, false>(a, (int)((code >> 16) & 0xFFFFu), (int)(code & 0xFFFFu),
+            gram_tile64<MODE, false, 1>(a, (int)((code >> 16) & 0xFFFFu), (int)(code & 0xFFFFu),
                               *reinterpret_cast<uint4(*)[3][4][kPanelBlockU4]>(lds), thread);
             __syncthreads();  // the next item reuses the LDS
         } else {

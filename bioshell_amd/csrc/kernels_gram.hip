@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
     const int t = (wg % kXcd) * per + (wg / kXcd);
     if (t >= a.n_tiles) return;
     const uint32_t code = a.tiles[t];
-    gram_tile64<MODE, true>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds, (int)threadIdx.x);
+    gram_tile64<MODE, true, 2>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds, (int)threadIdx.x);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n)
