@@ -16,6 +16,7 @@ from .crmsd import (  # noqa: F401
     crmsd,
     crmsd_all_vs_all,
     crmsd_all_vs_all_device,
+    crmsd_all_vs_all_row_stats_device,
     crmsd_all_vs_all_streamed,
     crmsd_batch,
     crmsd_batch_device,

@@ -86,6 +86,8 @@ SIGNATURES = {
     "bs_crmsd_all_vs_all_device": (c_status, [c_vp, c_vp, c_size, c_size, ctypes.c_int, c_vp, ctypes.c_int]),
     "bs_crmsd_all_vs_all_device_noalloc": (c_status, [c_vp, c_vp, c_size, c_size, ctypes.c_int, c_vp,
                                                       ctypes.c_int]),
+    "bs_crmsd_all_vs_all_row_stats_device": (c_status, [c_vp, c_vp, c_size, c_size, ctypes.c_int, c_vp, c_vp,
+                                                        c_vp]),
     "bs_crmsd_all_vs_all_streamed": (c_status, [c_vp, c_dp, c_size, c_size, ctypes.c_int, c_size, c_fp]),
     "bs_crmsd_batch": (c_status, [c_vp, ctypes.POINTER(c_dp), ctypes.POINTER(c_size), ctypes.POINTER(c_size),
                                   c_size, ctypes.c_int, ctypes.POINTER(c_fp)]),

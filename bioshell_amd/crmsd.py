@@ -281,6 +281,21 @@ def crmsd_all_vs_all_device(d_xyz: int, n_conf: int, n_atoms: int, d_out: int,
                                                    int(method), ctypes.c_void_p(d_out), int(out_f64)))
 
 
+def crmsd_all_vs_all_row_stats_device(d_xyz: int, n_conf: int, n_atoms: int, d_out: int, d_row_min: int,
+                                      d_row_max: int, method: Method = Method.BF16X3_GRAM,
+                                      ctx: Context | None = None) -> None:
+    """All-vs-all on device pointers with f32 output, and in the same call the
+    row minima and maxima of that matrix (what ``bs_rmsd_row_stats_device``
+    gives for it, bit for bit; ``retrieve_outliers`` and ``medoid_by_min_max``,
+    hierarchical.rs:198-217 and 105-131, read them).  The Gram methods take
+    them in the kernel's epilogue.  Asynchronous on the context's stream.
+    """
+    ctx = ctx or default_context()
+    check(_native.lib().bs_crmsd_all_vs_all_row_stats_device(
+        ctx.handle, ctypes.c_void_p(d_xyz), n_conf, n_atoms, int(method), ctypes.c_void_p(d_out),
+        ctypes.c_void_p(d_row_min), ctypes.c_void_p(d_row_max)))
+
+
 def crmsd_all_vs_all_streamed(models, method: Method = Method.BF16X3_GRAM, tile: int = 4096,
                               ctx: Context | None = None) -> RmsdMatrix:
     """All-vs-all streamed in tile x tile blocks to a host f32 triangle."""

@@ -308,6 +308,23 @@ bs_status bs_crmsd_all_vs_all_device(bs_context* ctx, const double* d_xyz, size_
     });
 }
 
+bs_status bs_crmsd_all_vs_all_row_stats_device(bs_context* ctx, const double* d_xyz, size_t n_conf,
+                                               size_t n_atoms, bs_method method, float* d_out_upper,
+                                               float* d_row_min, float* d_row_max)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        check_shape(n_conf, n_atoms);
+        BS_REQUIRE(valid_method(method), "unknown method %d", (int)method);
+        BS_REQUIRE(n_conf >= 2, "row statistics need at least 2 conformations");
+        BS_REQUIRE(d_xyz && d_out_upper && d_row_min && d_row_max, "null pointer argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        avsa_device(ctx, d_xyz, (int)n_conf, (int)n_atoms, method, d_out_upper, false, d_row_min,
+                    d_row_max);
+    });
+}
+
 bs_status bs_crmsd_all_vs_all(bs_context* ctx, const double* xyz, size_t n_conf, size_t n_atoms,
                               bs_method method, void* out_upper, int out_f64)
 {

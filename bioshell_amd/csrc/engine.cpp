@@ -225,15 +225,24 @@ void run_gram_with(const PrepView& A, const PrepView& B, const GramRun& run, con
     g.flag_cap = list_cap;
     g.lim = epilogue_limits(gram_thresholds(A.mode, A.L, A.n_kblk), A.L);
     g.max_wg = max_wg > 0 ? max_wg : 512;
+    const bool stats = run.stats_part != nullptr;
+    BS_REQUIRE(!stats || (run.triangle && !run.out_f64 && run.st_min && run.st_max),
+               "row statistics need the f32 triangle");
+    g.stats_part = run.stats_part;
+    g.stats_T = (A.n + kTileRows - 1) / kTileRows;
     (void)n_words;
 
     if (rec) rec->mark({kBPrep1, kBMain0}, s);
     launch_gram(g, A.mode, s);
     if (rec) rec->mark({kBMain1, kBFallback0}, s);
     BS_HIP(hipGetLastError());
+    // the statistics of the pairs the epilogue kept, as keys; the resolve
+    // kernel then adds the re-evaluated pairs, and the keys become f32
+    if (stats) launch_row_stats_combine(run.stats_part, A.n, run.st_min, run.st_max, s);
     launch_flag_resolve(A.xyz, A.cent, B.xyz, B.cent, A.L, ctl, list, list_cap, bits, (int64_t)n_out,
                         g.triangle, A.n, run.out_ld, run.d_out, g.out_f64, rec ? rec->h_count : nullptr,
-                        run.h_count, s);
+                        run.h_count, run.st_min, run.st_max, s);
+    if (stats) launch_row_stats_decode(run.st_min, run.st_max, A.n, s);
     BS_HIP(hipGetLastError());
     if (rec) {
         if (run.ends_total)
@@ -330,9 +339,11 @@ const uint32_t* GramScratch::tile_order(int tiles_r, int tiles_c, bool triangle,
 }
 
 void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m, void* d_out,
-                 bool out_f64)
+                 bool out_f64, float* d_rmin, float* d_rmax)
 {
     hipStream_t s = ctx->stream;
+    BS_REQUIRE(!d_rmin || (d_rmax && !out_f64 && n >= 2), "row statistics need the f32 triangle");
+    const bool fused = d_rmin && is_gram(m) && fused_stats_tiles(n) <= (size_t)kFusedStatsMaxTiles;
     RunRecord* rec = ctx->begin_record();
     rec->base.pairs = n >= 2 ? (uint64_t)n * (n - 1) / 2 : 0;
     if (n >= 2 && !is_gram(m)) {
@@ -349,12 +360,23 @@ void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m
         run.d_out = d_out;
         run.out_f64 = out_f64;
         run.ends_total = true;
+        if (fused) {
+            run.stats_part = static_cast<unsigned*>(
+                ctx->ws_stats.ensure(fused_stats_tiles(n) * 256 * 4, "row statistics tiles"));
+            run.st_min = reinterpret_cast<unsigned*>(d_rmin);
+            run.st_max = reinterpret_cast<unsigned*>(d_rmax);
+        }
         run_gram(view_of(ctx->prep), view_of(ctx->prep), run, ctx->gsc, s, rec);
         rec->base.main_launches = 1;
         rec->base.tile_m = kTileRows;
         rec->base.tile_n = kTileRows;
     } else {
         rec->mark({kBTotal0, kBTotal1, kBPrep0, kBPrep1, kBMain0, kBMain1, kBFallback0, kBFallback1}, s);
+    }
+    if (d_rmin && !fused) {
+        void* scratch = ctx->ws_stats.ensure(row_stats_scratch_bytes(n), "row statistics tiles");
+        launch_row_stats(static_cast<const float*>(d_out), n, d_rmin, d_rmax, scratch, s);
+        BS_HIP(hipGetLastError());
     }
 }
 

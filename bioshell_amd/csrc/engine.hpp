@@ -105,6 +105,12 @@ struct GramRun {
     bool out_f64 = false;
     int64_t out_ld = 0;
     bool ends_total = false;      // the record's total ends with this run
+    // row statistics from the epilogue (triangle, f32 output): tile results
+    // (fused_stats_tiles(n) x 1 KiB) and the n keys of each statistic, which
+    // the run leaves decoded to f32
+    unsigned* stats_part = nullptr;
+    unsigned* st_min = nullptr;
+    unsigned* st_max = nullptr;
 };
 
 // Scratch a Gram run needs besides the prepared sets.
@@ -210,9 +216,13 @@ size_t query_workspace_bytes(int n_q, int n_lib, int L, int k);
 GramMode mode_of(bs_method m);
 bool is_gram(bs_method m);
 
-// All-vs-all on device-resident coordinates; fills ctx->last.
+// All-vs-all on device-resident coordinates; fills ctx->last.  With d_rmin
+// set (f32 output), also the row minima and maxima of the triangle
+// (launch_row_stats' results): from the Gram epilogue where the method has one
+// and the triangle has at most kFusedStatsMaxTiles tiles, else by
+// launch_row_stats over the written triangle.
 void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m, void* d_out,
-                 bool out_f64);
+                 bool out_f64, float* d_rmin = nullptr, float* d_rmax = nullptr);
 
 }  // namespace bs
 

@@ -26,11 +26,21 @@ constexpr int epilogue_lds_bytes()
 // a pair whose S = -B is not positive definite or whose bracket stays wide
 // is flagged, like every pair the error model cannot vouch for, and is
 // re-evaluated in FP64 from the coordinates.
-template <int TILE, int P>
+// LDS bytes of the epilogue's row-statistics keys (64 x 64 tile, rows padded to 65).
+constexpr int kStatsKeysBytes = 64 * 65 * 4;
+
+// STATS (64 x 64 tiles of a triangle, f32 output): every pair also leaves
+// the order-preserving key of its stored value in an LDS tile (all-ones for a
+// pair outside the triangle or flagged: the resolve kernel adds the flagged
+// pairs' final values), and the work-group folds the tile's rows and columns
+// into the 256 keys of row_stats_tile_kernel's layout at a.stats_part.
+template <int TILE, int P, bool STATS>
 __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x16 (&acc)[9],
                                          char* lds, int thread)
 {
     static_assert(P == 1 || P == 2, "pairs per block: 1 or 2 (rows of one group of four)");
+    static_assert(!STATS || TILE == 64, "row statistics: 64 x 64 tiles");
+    unsigned* skeys = reinterpret_cast<unsigned*>(lds + epilogue_lds_bytes<TILE>());
     constexpr int kWaves = (TILE / 32) * (TILE / 32);
     constexpr int kThreads = 64 * kWaves;
     const int lane = thread & 63;
@@ -114,7 +124,11 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
                               (err * err > lim.tau2L * o[p].rmsd2_L) || (c[p][9] + cj[9] > lim.rep_h_max);
             const int64_t oi =
                 a.triangle ? tri_index(a.n_rows, i + p, j) : (int64_t)(i + p) * a.out_ld + j;
-            if (ok) {
+            if constexpr (STATS) {  // f32 output
+                const float v = __builtin_amdgcn_sqrtf((float)(o[p].rmsd2_L * lim.inv_L));
+                if (ok) static_cast<float*>(a.out)[oi] = v;
+                skeys[(il + p) * 65 + jl] = ok && !flag && v == v ? fkey(__float_as_uint(v)) : 0xFFFFFFFFu;
+            } else if (ok) {
                 if (a.out_f64)
                     static_cast<double*>(a.out)[oi] = sqrt(o[p].rmsd2_L * lim.inv_L);
                 else  // v_sqrt_f32 (1 ulp) instead of the 17-instruction correctly rounded sqrtf
@@ -132,6 +146,36 @@ __device__ __forceinline__ void epilogue(const GramArgs& a, int tr, int tc, f32x
             if (!((flagged >> r) & 1u)) continue;
             const int i = tr * TILE + il0 + 8 * (r / 4) + (r % 4);
             flag_output(a, true, a.triangle ? tri_index(a.n_rows, i, j) : (int64_t)i * a.out_ld + j);
+        }
+    }
+    if constexpr (STATS) {
+        __syncthreads();  // every wave's keys are in LDS
+        // tile row r and tile column r: four threads of 16 entries each, then two shuffles
+        const int r = thread >> 2, q = thread & 3;
+        unsigned lo = 0xFFFFFFFFu, hi = 0u, clo = 0xFFFFFFFFu, chi = 0u;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const unsigned kr = skeys[r * 65 + q * 16 + c];
+            const unsigned kc = skeys[(q * 16 + c) * 65 + r];
+            lo = min(lo, kr);
+            hi = max(hi, kr == 0xFFFFFFFFu ? 0u : kr);
+            clo = min(clo, kc);
+            chi = max(chi, kc == 0xFFFFFFFFu ? 0u : kc);
+        }
+#pragma unroll
+        for (int m = 1; m <= 2; m <<= 1) {
+            lo = min(lo, (unsigned)__shfl_xor((int)lo, m));
+            hi = max(hi, (unsigned)__shfl_xor((int)hi, m));
+            clo = min(clo, (unsigned)__shfl_xor((int)clo, m));
+            chi = max(chi, (unsigned)__shfl_xor((int)chi, m));
+        }
+        if (q == 0) {
+            const int64_t b = (int64_t)tr * a.stats_T - (int64_t)tr * (tr - 1) / 2 + (tc - tr);
+            unsigned* out = a.stats_part + b * 256;
+            out[r] = lo;
+            out[64 + r] = hi;
+            out[128 + r] = clo;
+            out[192 + r] = chi;
         }
     }
 }
@@ -291,8 +335,8 @@ __device__ __forceinline__ void mfma_cross(const PlaneRegs& h, const PlaneRegs& 
 // the other register set, and runs the 27 (or 72) MFMAs of block k.
 //
 // PAIRS: pairs per block of the epilogue, 2 where the registers allow (the
-// Gram kernel), 1 in the batch kernel.
-template <int MODE, bool SPLIT, int PAIRS>
+// Gram kernel), 1 in the batch kernel.  STATS: the epilogue's row statistics.
+template <int MODE, bool SPLIT, int PAIRS, bool STATS = false>
 __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
                                             uint4 (&lds)[3][4][kPanelBlockU4], int thread)
 {
@@ -411,7 +455,9 @@ __device__ __forceinline__ void gram_tile64(const GramArgs& a, int tr, int tc,
 #undef BS_STEP3
 
     __builtin_amdgcn_s_setprio(0);
-    epilogue<64, PAIRS>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
+    static_assert(!STATS || sizeof(lds) >= epilogue_lds_bytes<64>() + kStatsKeysBytes,
+                  "the statistics keys fit in the ring's LDS");
+    epilogue<64, PAIRS, STATS>(a, tr, tc, acc, reinterpret_cast<char*>(&lds[0][0][0]), thread);
 }
 
 // One 32 x 32 tile of pairs by a single wave: operand fragments are read
@@ -437,7 +483,7 @@ __device__ __forceinline__ void gram_tile32(const GramArgs& a, int tr, int tc, c
     for (int kb = 0; kb < a.n_kblk; ++kb)
         mfma_block<MODE>(A + (int64_t)kb * kPanelBlockU4, B + (int64_t)kb * kPanelBlockU4, lane, acc);
     __builtin_amdgcn_s_setprio(0);
-    epilogue<32, 1>(a, tr, tc, acc, lds, thread);
+    epilogue<32, 1, false>(a, tr, tc, acc, lds, thread);
 }
 
 }  // namespace bs

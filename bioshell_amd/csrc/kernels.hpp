@@ -123,6 +123,12 @@ struct GramArgs {
     EpilogueLimits lim;     // flagging limits and the RMSD scale
     // launch shape
     int max_wg;             // resident work-groups (2 per CU)
+    // Row statistics from the epilogue (triangle, f32 output): when set, tile
+    // (tr, tc) writes the 256 keys of row_stats_tile_kernel's layout to
+    // stats_part + 256 (tr stats_T - tr (tr - 1) / 2 + tc - tr), leaving out
+    // the flagged pairs (the resolve kernel adds their final values).
+    unsigned* stats_part;
+    int stats_T;            // tiles per side of the triangle
 };
 
 // Builds the logical tile order (XCD-aware super-blocks) on the host.
@@ -140,10 +146,14 @@ void launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 // and zeroes count and done, so the region is ready for the next launch.  It
 // also stores the count into h_report_a / h_report_b when given (pinned host
 // words, written with a system-scope store: no separate device-to-host copy).
+// With st_min / st_max set (triangle, f32 output), each re-evaluated value
+// also goes into the row-statistics keys of both its elements (atomic
+// min / max on order-preserving keys).
 void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double* xyz_b,
                          const double* cent_b, int L, uint32_t* ctl, const int64_t* list, int cap,
                          uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld, void* out,
-                         int out_f64, uint32_t* h_report_a, uint32_t* h_report_b, hipStream_t s);
+                         int out_f64, uint32_t* h_report_a, uint32_t* h_report_b, unsigned* st_min,
+                         unsigned* st_max, hipStream_t s);
 // Queues the set bits of bits[0 .. n_words) as bit indices (word * 32 + bit)
 // into list (cap entries), clearing the queued ones; *count += set bits.
 void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,
@@ -223,6 +233,27 @@ void launch_batch_fallback(const BatchArgs& b, hipStream_t s);
 // counts); scratch holds row_stats_scratch_bytes(n).
 size_t row_stats_scratch_bytes(int n);
 void launch_row_stats(const float* up, int n, float* rmin, float* rmax, void* scratch, hipStream_t s);
+// Order-preserving unsigned keys of f32 bits: every non-NaN float, negative
+// ones included, keeps its order.  All-ones is the key of a NaN only, so it
+// serves as "no value" for a minimum.
+__host__ __device__ inline unsigned fkey(unsigned bits) { return (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u); }
+__host__ __device__ inline unsigned fkey_inverse(unsigned key) { return (key & 0x80000000u) ? (key ^ 0x80000000u) : ~key; }
+constexpr unsigned kKeyFltMax = 0xFF7FFFFFu;     // fkey(bits of f32::MAX)
+constexpr unsigned kKeyNegFltMax = 0x00800000u;  // fkey(bits of -f32::MAX)
+
+// Row statistics taken in the Gram epilogue: the tile results of the whole
+// triangle at once (1 KiB per tile), for at most kFusedStatsMaxTiles tiles
+// (n up to about 23k; larger triangles take launch_row_stats afterwards).
+constexpr int kFusedStatsMaxTiles = 1 << 16;
+inline size_t fused_stats_tiles(int n)
+{
+    const size_t T = (size_t)(n + 63) / 64;
+    return T * (T + 1) / 2;
+}
+// Combines the epilogue's tile results into keys (kmin / kmax, n each, n >= 2),
+// which the resolve kernel then updates; decode turns the keys into f32.
+void launch_row_stats_combine(const unsigned* part, int n, unsigned* kmin, unsigned* kmax, hipStream_t s);
+void launch_row_stats_decode(unsigned* kmin, unsigned* kmax, int n, hipStream_t s);
 // For each of m members, the largest distance to the other members.
 void launch_members_row_max(const float* up, int n, const unsigned* members, int m, float* out,
                             hipStream_t s);

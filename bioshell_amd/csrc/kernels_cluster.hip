@@ -36,12 +36,8 @@ __global__ void fill_u32_kernel(unsigned* p, unsigned v, int64_t n)
 }
 
 // Row statistics compare f32 values through order-preserving unsigned keys
-// (every non-NaN float, negative ones included, keeps its order); NaN entries
-// are skipped, as the reference's `d < min` / `d > max` tests skip them.
-__device__ inline unsigned fkey(unsigned bits) { return (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u); }
-__device__ inline unsigned fkey_inverse(unsigned key) { return (key & 0x80000000u) ? (key ^ 0x80000000u) : ~key; }
-constexpr unsigned kKeyFltMax = 0xFF7FFFFFu;     // fkey(bits of f32::MAX)
-constexpr unsigned kKeyNegFltMax = 0x00800000u;  // fkey(bits of -f32::MAX)
+// (fkey in kernels.hpp); NaN entries are skipped, as the reference's
+// `d < min` / `d > max` tests skip them.
 
 // First block of tile row r in the row-by-row order of the T x T upper
 // triangle of tiles (tr <= tc).
@@ -161,6 +157,15 @@ __global__ __launch_bounds__(512) void row_stats_reduce_kernel(const unsigned* _
         rmin[e] = last ? fkey_inverse(lo) : lo;
         rmax[e] = last ? fkey_inverse(hi) : hi;
     }
+}
+
+// Keys of elements [0, n) to f32 values, in place.
+__global__ void row_stats_decode_kernel(unsigned* __restrict__ kmin, unsigned* __restrict__ kmax, int n)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    kmin[e] = fkey_inverse(kmin[e]);
+    kmax[e] = fkey_inverse(kmax[e]);
 }
 
 // Member a of the list: the largest distance to the other members (gather).
@@ -533,6 +538,17 @@ void launch_row_stats(const float* up, int n, float* rmin, float* rmax, void* sc
         row_stats_reduce_kernel<<<T, 512, 0, s>>>(part, n, T, t0, t1, base, t0 == 0, t1 == T, a, b);
         t0 = t1;
     }
+}
+
+void launch_row_stats_combine(const unsigned* part, int n, unsigned* kmin, unsigned* kmax, hipStream_t s)
+{
+    const int T = (n + 63) / 64;
+    row_stats_reduce_kernel<<<T, 512, 0, s>>>(part, n, T, 0, T, 0, 1, 0, kmin, kmax);
+}
+
+void launch_row_stats_decode(unsigned* kmin, unsigned* kmax, int n, hipStream_t s)
+{
+    row_stats_decode_kernel<<<(n + 255) / 256, 256, 0, s>>>(kmin, kmax, n);
 }
 
 void launch_members_row_max(const float* up, int n, const unsigned* members, int m, float* out,

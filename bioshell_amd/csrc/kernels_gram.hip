@@ -31,7 +31,7 @@ namespace {
 // second resident batch (ids [n_cu, 2 n_cu)) start about half a tile late, so
 // that on every SIMD one work-group's MFMA phase overlaps the other's FP64
 // epilogue (VALU).  Tiles cost the same, so refilled slots keep the offset.
-template <int MODE>
+template <int MODE, bool STATS>
 __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
 {
     __shared__ uint4 lds[3][4][kPanelBlockU4];  // 3 stages x (A0, A1, B0, B1) x 6 KiB
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256, 2) void gram_kernel(GramArgs a)
     const int t = (wg % kXcd) * per + (wg / kXcd);
     if (t >= a.n_tiles) return;
     const uint32_t code = a.tiles[t];
-    gram_tile64<MODE, true, 2>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds, (int)threadIdx.x);
+    gram_tile64<MODE, true, 2, STATS>(a, (int)(code >> 16), (int)(code & 0xFFFFu), lds, (int)threadIdx.x);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n)
@@ -112,21 +112,31 @@ __device__ inline void decode_output(int64_t o, int triangle, int n, int64_t out
     }
 }
 
-// FP64 re-evaluation of one flagged output o (lane 0 stores the result).
+// FP64 re-evaluation of one flagged output o (lane 0 stores the result and,
+// with st_min set, folds the stored f32 value into both elements' keys).
 __device__ inline void resolve_output(int64_t o, const double* __restrict__ xa, const double* __restrict__ ca,
                                       const double* __restrict__ xb, const double* __restrict__ cb, int L,
                                       int triangle, int n, int64_t out_ld, void* __restrict__ out, int out_f64,
-                                      int lane)
+                                      unsigned* st_min, unsigned* st_max, int lane)
 {
     int i, j;
     decode_output(o, triangle, n, out_ld, i, j);
     const double r = fallback_pair(xa + (int64_t)i * L * 3, ca + 3 * (int64_t)i, xb + (int64_t)j * L * 3,
                                    cb + 3 * (int64_t)j, L, lane);
     if (lane == 0) {
-        if (out_f64)
+        if (out_f64) {
             static_cast<double*>(out)[o] = r;
-        else
-            static_cast<float*>(out)[o] = (float)r;
+        } else {
+            const float v = (float)r;
+            static_cast<float*>(out)[o] = v;
+            if (st_min && v == v) {
+                const unsigned k = fkey(__float_as_uint(v));
+                atomicMin(st_min + i, k);
+                atomicMin(st_min + j, k);
+                atomicMax(st_max + i, k);
+                atomicMax(st_max + j, k);
+            }
+        }
     }
 }
 
@@ -145,7 +155,8 @@ __global__ __launch_bounds__(256) void flag_resolve_kernel(const double* __restr
                                                            uint32_t* __restrict__ bits, int64_t n_out,
                                                            int triangle, int n, int64_t out_ld,
                                                            void* __restrict__ out, int out_f64,
-                                                           uint32_t* h_report_a, uint32_t* h_report_b)
+                                                           uint32_t* h_report_a, uint32_t* h_report_b,
+                                                           unsigned* st_min, unsigned* st_max)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t cnt = ctl[0];  // read before this group counts itself done
@@ -154,7 +165,8 @@ __global__ __launch_bounds__(256) void flag_resolve_kernel(const double* __restr
     const int64_t m = min((int64_t)cnt, (int64_t)cap);
     for (int64_t k = wave; k < m; k += n_waves) {
         const int64_t o = list[k];
-        if (o >= 0 && o < n_out) resolve_output(o, xa, ca, xb, cb, L, triangle, n, out_ld, out, out_f64, lane);
+        if (o >= 0 && o < n_out)
+            resolve_output(o, xa, ca, xb, cb, L, triangle, n, out_ld, out, out_f64, st_min, st_max, lane);
     }
     if ((int64_t)cnt > (int64_t)cap) {  // uniform: the bitmap holds the rest
         const int64_t n_words = (n_out + 31) >> 5;
@@ -171,7 +183,7 @@ __global__ __launch_bounds__(256) void flag_resolve_kernel(const double* __restr
                 const int64_t o = (w0 + src) * 32 + bit;
                 if (lane == src) word &= word - 1u;  // clear the lowest set bit
                 if (o >= n_out) continue;
-                resolve_output(o, xa, ca, xb, cb, L, triangle, n, out_ld, out, out_f64, lane);
+                resolve_output(o, xa, ca, xb, cb, L, triangle, n, out_ld, out, out_f64, st_min, st_max, lane);
             }
         }
     }
@@ -206,10 +218,15 @@ void launch_gram(const GramArgs& a, GramMode mode, hipStream_t s)
     if (a.n_tiles <= 0) return;
     const int per = (a.n_tiles + kXcd - 1) / kXcd;
     const int grid = per * kXcd;
-    if (mode == kGramBF16x3)
-        gram_kernel<kGramBF16x3><<<grid, 256, 0, s>>>(a);
+    const bool st = a.stats_part != nullptr;
+    if (mode == kGramBF16x3 && st)
+        gram_kernel<kGramBF16x3, true><<<grid, 256, 0, s>>>(a);
+    else if (mode == kGramBF16x3)
+        gram_kernel<kGramBF16x3, false><<<grid, 256, 0, s>>>(a);
+    else if (st)
+        gram_kernel<kGramFP32, true><<<grid, 256, 0, s>>>(a);
     else
-        gram_kernel<kGramFP32><<<grid, 256, 0, s>>>(a);
+        gram_kernel<kGramFP32, false><<<grid, 256, 0, s>>>(a);
 }
 
 void launch_zero_u32(uint32_t* p, int n, hipStream_t s)
@@ -228,7 +245,8 @@ void launch_zero_words(uint32_t* p, int64_t n_words, hipStream_t s)
 void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double* xyz_b,
                          const double* cent_b, int L, uint32_t* ctl, const int64_t* list, int cap,
                          uint32_t* bits, int64_t n_out, int triangle, int n, int64_t out_ld, void* out,
-                         int out_f64, uint32_t* h_report_a, uint32_t* h_report_b, hipStream_t s)
+                         int out_f64, uint32_t* h_report_a, uint32_t* h_report_b, unsigned* st_min,
+                         unsigned* st_max, hipStream_t s)
 {
     // 1024 waves: a typical list (tens to thousands of pairs) is one pass, a
     // full list or a bitmap scan takes several.  A grid sized for the bitmap
@@ -236,7 +254,7 @@ void launch_flag_resolve(const double* xyz_a, const double* cent_a, const double
     // done-counter atomics of the idle groups.
     flag_resolve_kernel<<<256, 256, 0, s>>>(
         xyz_a, cent_a, xyz_b, cent_b, L, ctl, list, cap, bits, n_out, triangle, n, out_ld, out, out_f64,
-        h_report_a, h_report_b);
+        h_report_a, h_report_b, st_min, st_max);
 }
 
 void launch_flag_compact(uint32_t* bits, int64_t n_words, int64_t* list, int cap, uint32_t* count,

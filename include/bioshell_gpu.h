@@ -182,6 +182,19 @@ bs_status bs_crmsd_all_vs_all_device(bs_context* ctx, const double* d_xyz, size_
                                      size_t n_atoms, bs_method method, void* d_out_upper,
                                      int out_f64);
 
+/* bs_crmsd_all_vs_all_device with f32 output, and in the same call the row
+ * minimum and maximum over j != i of the matrix it writes: for each of the
+ * n_conf elements, bit for bit what bs_rmsd_row_stats_device returns for
+ * d_out_upper afterwards (retrieve_outliers and medoid_by_min_max,
+ * hierarchical.rs:198-217 and 105-131, read these).  The Gram methods take
+ * them from the kernel's epilogue, so the triangle is not read back; the
+ * plain methods, and triangles of more than 65,536 64 x 64 tiles (n_conf
+ * above about 23,000), run the row-statistics kernels over d_out_upper.
+ * n_conf >= 2.  Device buffers, asynchronous on the context's stream. */
+bs_status bs_crmsd_all_vs_all_row_stats_device(bs_context* ctx, const double* d_xyz, size_t n_conf,
+                                               size_t n_atoms, bs_method method, float* d_out_upper,
+                                               float* d_row_min, float* d_row_max);
+
 /* The same as bs_crmsd_all_vs_all_device, without allocating anything.  No
  * counterpart in the reference: it is for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -161,6 +161,83 @@ def test_row_stats_in_chunks_of_tile_rows(gpu_ctx, monkeypatch, chunk):
     assert np.array_equal(lo, wlo) and np.array_equal(hi, whi)
 
 
+def fused_matrix_and_stats(ctx, x, method):
+    """bs_crmsd_all_vs_all_row_stats_device on torch device buffers."""
+    import torch
+    n, L, _ = x.shape
+    dx = torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+    out = torch.full((n * (n - 1) // 2,), -1.0, dtype=torch.float32, device="cuda:0")
+    lo = torch.full((n,), -2.0, dtype=torch.float32, device="cuda:0")
+    hi = torch.full((n,), -2.0, dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    B.crmsd_all_vs_all_row_stats_device(dx.data_ptr(), n, L, out.data_ptr(), lo.data_ptr(), hi.data_ptr(),
+                                        method, ctx=ctx)
+    ctx.synchronize()
+    return out.cpu().numpy(), lo.cpu().numpy(), hi.cpu().numpy()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("n,L", [(2, 30), (3, 12), (63, 40), (64, 300), (65, 40), (130, 77), (777, 45)])
+@pytest.mark.parametrize("method", [B.Method.BF16X3_GRAM, B.Method.FP32_GRAM, B.Method.FP32_PLAIN])
+def test_row_stats_fused_into_all_vs_all(gpu_ctx, n, L, method):
+    """The matrix of the fused call equals bs_crmsd_all_vs_all_device's, and
+    its row statistics equal bs_rmsd_row_stats' (and the oracle's) on that
+    matrix, bit for bit.  5 % near-identical conformations put flagged pairs,
+    whose final values only the re-evaluation gives, among the row minima."""
+    x = B.generate_ensemble(700 + n, n, L, sigma=(0.5, 3.0), near_fraction=0.05)
+    up, lo, hi = fused_matrix_and_stats(gpu_ctx, x, method)
+    flagged = gpu_ctx.last_stats()["flagged_pairs"]
+    want_up = B.crmsd_all_vs_all(x, method, ctx=gpu_ctx).upper
+    assert np.array_equal(bits(up), bits(want_up))
+    slo, shi = C.row_stats(B.RmsdMatrix(n, up), ctx=gpu_ctx)
+    wlo, whi = O.row_stats(up, n)
+    assert np.array_equal(bits(lo), bits(slo)) and np.array_equal(bits(hi), bits(shi))
+    assert np.array_equal(bits(lo), bits(wlo)) and np.array_equal(bits(hi), bits(whi))
+    print(method.name, n, L, "flagged", flagged)
+
+
+def test_row_stats_fused_with_flagged_minima(gpu_ctx):
+    """Near-duplicate pairs are flagged and re-evaluated after the epilogue:
+    here every element's row minimum comes from such a pair (each has a copy
+    0.01 A away), so the fused minima are right only if the re-evaluated
+    values reach them."""
+    n2, L = 150, 60
+    base = B.generate_ensemble(41, n2, L, sigma=(1.0, 3.0))
+    rng = np.random.default_rng(41)
+    x = np.concatenate([base, base + rng.normal(scale=0.01, size=base.shape)])
+    up, lo, hi = fused_matrix_and_stats(gpu_ctx, x, B.Method.BF16X3_GRAM)
+    assert gpu_ctx.last_stats()["flagged_pairs"] >= n2
+    wlo, whi = O.row_stats(up, 2 * n2)
+    assert np.array_equal(bits(lo), bits(wlo)) and np.array_equal(bits(hi), bits(whi))
+    want = O.all_vs_all(x)
+    assert np.abs(up.astype(np.float64) - want).max() <= 1e-4
+
+
+def test_row_stats_fused_through_bitmap_scan(gpu_ctx):
+    """More flagged pairs than the re-evaluation queue holds (rigid copies of
+    two chains, 12 atoms): the bitmap scan's pairs reach the statistics too."""
+    x = np.concatenate([B.generate_ensemble(1, 1200, 12, sigma=0.0),
+                        B.generate_ensemble(2, 1200, 12, sigma=0.0)])
+    up, lo, hi = fused_matrix_and_stats(gpu_ctx, x, B.Method.BF16X3_GRAM)
+    assert gpu_ctx.last_stats()["flagged_pairs"] > (1 << 20)
+    slo, shi = C.row_stats(B.RmsdMatrix(x.shape[0], up), ctx=gpu_ctx)
+    assert np.array_equal(bits(lo), bits(slo)) and np.array_equal(bits(hi), bits(shi))
+
+
+def test_row_stats_fused_argument_errors(gpu_ctx):
+    import torch
+    d = torch.zeros(64, dtype=torch.float64, device="cuda:0")
+    f = torch.zeros(64, dtype=torch.float32, device="cuda:0")
+    with pytest.raises(B.BioshellGpuError):
+        B.crmsd_all_vs_all_row_stats_device(d.data_ptr(), 1, 4, f.data_ptr(), f.data_ptr(), f.data_ptr(),
+                                            ctx=gpu_ctx)
+    with pytest.raises(B.BioshellGpuError):
+        B.crmsd_all_vs_all_row_stats_device(d.data_ptr(), 2, 4, f.data_ptr(), 0, f.data_ptr(), ctx=gpu_ctx)
+
+
 def test_clustering_2000_timing(gpu_ctx):
     n, L = 2000, 100
     x = B.generate_ensemble(9, n, L, sigma=(0.5, 3.0))
