@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace bs {
@@ -343,7 +344,11 @@ void avsa_device(bs_context* ctx, const double* d_xyz, int n, int L, bs_method m
 {
     hipStream_t s = ctx->stream;
     BS_REQUIRE(!d_rmin || (d_rmax && !out_f64 && n >= 2), "row statistics need the f32 triangle");
-    const bool fused = d_rmin && is_gram(m) && fused_stats_tiles(n) <= (size_t)kFusedStatsMaxTiles;
+    // BIOSHELL_FUSED_STATS_MAX_TILES lowers the limit (tests of the other path)
+    size_t max_tiles = kFusedStatsMaxTiles;
+    if (const char* e = std::getenv("BIOSHELL_FUSED_STATS_MAX_TILES"))
+        max_tiles = std::min<size_t>(max_tiles, (size_t)std::max(0, std::atoi(e)));
+    const bool fused = d_rmin && is_gram(m) && fused_stats_tiles(n) <= max_tiles;
     RunRecord* rec = ctx->begin_record();
     rec->base.pairs = n >= 2 ? (uint64_t)n * (n - 1) / 2 : 0;
     if (n >= 2 && !is_gram(m)) {

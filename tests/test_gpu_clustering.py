@@ -227,6 +227,19 @@ def test_row_stats_fused_through_bitmap_scan(gpu_ctx):
     assert np.array_equal(bits(lo), bits(slo)) and np.array_equal(bits(hi), bits(shi))
 
 
+@pytest.mark.parametrize("limit", ["0", "10"])
+def test_row_stats_fused_above_the_tile_limit(gpu_ctx, monkeypatch, limit):
+    """Triangles beyond the epilogue's tile limit (lowered here) take the
+    row-statistics kernels over the written matrix: the same results."""
+    n, L = 777, 45
+    x = B.generate_ensemble(777, n, L, sigma=(0.5, 3.0), near_fraction=0.05)
+    up0, lo0, hi0 = fused_matrix_and_stats(gpu_ctx, x, B.Method.BF16X3_GRAM)
+    monkeypatch.setenv("BIOSHELL_FUSED_STATS_MAX_TILES", limit)
+    up, lo, hi = fused_matrix_and_stats(gpu_ctx, x, B.Method.BF16X3_GRAM)
+    assert np.array_equal(bits(up), bits(up0))
+    assert np.array_equal(bits(lo), bits(lo0)) and np.array_equal(bits(hi), bits(hi0))
+
+
 def test_row_stats_fused_argument_errors(gpu_ctx):
     import torch
     d = torch.zeros(64, dtype=torch.float64, device="cuda:0")
