@@ -13,6 +13,9 @@ constexpr int epilogue_lds_bytes()
     return 2 * TILE * 12 * 8;
 }
 
+// LDS bytes of the epilogue's row-statistics keys (64 x 64 tile, rows padded to 65).
+constexpr int kStatsKeysBytes = 64 * 65 * 4;
+
 // The FP64 QCP epilogue of a TILE x TILE tile whose accumulators the calling
 // waves hold: TILE 64 is four waves of a work-group (wave w owns rows
 // 32 (w/2) .. +31 and columns 32 (w%2) .. +31), TILE 32 is one wave on its own.
@@ -26,9 +29,7 @@ constexpr int epilogue_lds_bytes()
 // a pair whose S = -B is not positive definite or whose bracket stays wide
 // is flagged, like every pair the error model cannot vouch for, and is
 // re-evaluated in FP64 from the coordinates.
-// LDS bytes of the epilogue's row-statistics keys (64 x 64 tile, rows padded to 65).
-constexpr int kStatsKeysBytes = 64 * 65 * 4;
-
+//
 // STATS (64 x 64 tiles of a triangle, f32 output): every pair also leaves
 // the order-preserving key of its stored value in an LDS tile (all-ones for a
 // pair outside the triangle or flagged: the resolve kernel adds the flagged
